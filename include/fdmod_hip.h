@@ -1,0 +1,140 @@
+/*
+ * fdmod_hip.h -- C-ABI of libfdmod_hip, the MI355X (gfx950) implementation of the
+ * TTI-FDModeling rotated-staggered-grid (RSG) 2-D TTI elastic modelling loop.
+ *
+ * The reference has no library boundary for this path: the loop is inline in each
+ * program's main().  These entry points replace the following reference code, so
+ * that the three host programs (coordinate rotation, tensor rotation, Bond
+ * transformation) keep their getpar surface, model files and SU I/O and hand the
+ * work between "model read" and "trace write" to the GPU:
+ *
+ *   fdmod_create         grid sizing, BT:296-340 (N=5, ealThick+=N, Nz, Nx)
+ *   fdmod_set_model      padding BT:452-457 (padModel2d, L/absorbc/padmod.c:33-58),
+ *                        stability BT:472 (L/fdmod/stable.c:217-242),
+ *                        MEAL BT:474-475 (L/absorbc/meal.c:51-201, 450-491),
+ *                        Thomsen->Cij(theta) BT:493-526 / TR:483-503 / CR:497-514,
+ *                        invrho / trig averaging BT:527-542 / CR:516-535,
+ *                        Bond free-surface fix-up BT:543-568
+ *   fdmod_set_source     source geometry after read2dsource (L/fdmod/obsinfo.c:34-87)
+ *   fdmod_set_receivers  receiver geometry after read2dreceiver (obsinfo.c:175-230)
+ *   fdmod_set_snapshots  snapshot cadence, BT:916-919 (L/fdmod/isoelastic.c:240-289)
+ *   fdmod_set_energy     energy diagnostic switch, BT:926-952
+ *   fdmod_run            the time loop BT:641-954 (TR:604-935, CR:601-926)
+ *   fdmod_get_traces     Receiver2D.recv after the loop, handed to traces2d_output
+ *                        (L/fdmod/traces.c:146-196)
+ *   fdmod_get_array      the side-effect dumps BT:476-491, 569-596 and the fields
+ *
+ * Citations: BT/TR/CR = src/main/sufdmod2dTTIelastic/sufdmod2dTTIelastic_{bond_
+ * transformation,tensor_rotation,coordinate_rotation}.c of the reference.
+ *
+ * Conventions.  All arrays cross the boundary as host pointers in the reference's
+ * layout (CWP alloc2float(n1=nz, n2=nx): nx records of nz floats, iz fastest).  The
+ * library copies inputs into device memory it owns and never keeps a host pointer
+ * past a call.  Every entry point returns 0 or a negative FDMOD_E* code; the message
+ * of the last failure is available from fdmod_last_error().  Argument validation
+ * (including the stability check) runs on the host before any device work.  One
+ * context belongs to one host thread; calls on one context are not re-entrant.
+ */
+#ifndef FDMOD_HIP_H
+#define FDMOD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDMOD_OK 0
+#define FDMOD_EINVAL (-1)   /* bad argument (sizes, flags, NULL pointers)           */
+#define FDMOD_EUNSTABLE (-2) /* dt larger than the stability limit (stable.c:236)   */
+#define FDMOD_ERANGE (-3)   /* source/receiver outside the model (obsinfo.c:69-80)  */
+#define FDMOD_EDEVICE (-4)  /* HIP runtime failure                                  */
+#define FDMOD_ENOMEM (-5)   /* device or host allocation failed                     */
+#define FDMOD_ESTATE (-6)   /* call out of order (e.g. run before set_model)        */
+
+/* rotation forms: the three programs under src/main/sufdmod2dTTIelastic/ */
+#define FDMOD_FORM_COORD 0  /* sufdmod2dTTIelastic_coordinate_rotation */
+#define FDMOD_FORM_TENSOR 1 /* sufdmod2dTTIelastic_tensor_rotation     */
+#define FDMOD_FORM_BOND 2   /* sufdmod2dTTIelastic_bond_transformation */
+
+/* trace components, the flag values of traces2d_output (traces.c:166-190) */
+#define FDMOD_TRACE_VX 1
+#define FDMOD_TRACE_VZ 3
+#define FDMOD_TRACE_P 4
+
+/* array ids for fdmod_get_array / fdmod_set_array (padded Nx*Nz, iz fastest) */
+enum fdmod_array {
+  FDMOD_A_VP = 0, FDMOD_A_VS, FDMOD_A_RHO, FDMOD_A_EPS, FDMOD_A_DELTA, FDMOD_A_THETA,
+  FDMOD_A_INVRHO, FDMOD_A_C11, FDMOD_A_C13, FDMOD_A_C15, FDMOD_A_C33, FDMOD_A_C35, FDMOD_A_C55,
+  FDMOD_A_SIN2, FDMOD_A_COS2, FDMOD_A_SIN2T, FDMOD_A_SIN, FDMOD_A_COS, FDMOD_A_SINV, FDMOD_A_COSV,
+  FDMOD_A_D, FDMOD_A_BETA, FDMOD_A_COEF1, FDMOD_A_COEF2,
+  FDMOD_A_VX, FDMOD_A_VZ, FDMOD_A_SXX, FDMOD_A_SZZ, FDMOD_A_SXZ, FDMOD_A_TXX, FDMOD_A_TZZ, FDMOD_A_TXZ,
+  FDMOD_A_COUNT
+};
+
+/* getpar parameters of the three programs that shape the run (BT:134-249). */
+typedef struct fdmod_params {
+  int form;        /* FDMOD_FORM_*                                                 */
+  int nz, nx, nt;  /* model samples (1st/2nd dimension) and time samples           */
+  float dx, dz, dt;
+  int ealThick;    /* absorbing layers as given by the user; N=5 is added inside   */
+  float eal_alpha; /* -1 selects the library default 2.5 (meal.c:74-77)            */
+  int freesurface; /* 0/1                                                          */
+  float fpeak;     /* MEAL reference frequency                                     */
+  float scalar;    /* applied to theta (degrees)                                   */
+  int mode;        /* 1: it = 0..nt-1; 2: it = nt-1..0 (BT:651)                    */
+} fdmod_params;
+
+typedef struct fdmod_ctx fdmod_ctx;
+
+/* Snapshot sink: called on the host at each snapshot time with one cropped field
+ * (kind "vx", "vz" or "p"; n1 = samples per record (z), n2 = records (x)).  The data
+ * pointer is valid for the duration of the call.  Return nonzero to abort the run. */
+typedef int (*fdmod_snapshot_fn)(void *user, int it, const char *kind, const float *data, int n1, int n2);
+
+int fdmod_create(const fdmod_params *p, fdmod_ctx **out);
+void fdmod_destroy(fdmod_ctx *ctx);
+const char *fdmod_last_error(const fdmod_ctx *ctx);
+const char *fdmod_strerror(int code);
+
+/* out[0..6] = Nz, Nx, ealThick (effective), izStart, izEnd, ixStart, ixEnd */
+int fdmod_dims(const fdmod_ctx *ctx, int *out7);
+
+/* Unpadded nz*nx arrays in file order; theta in degrees.  vs and rho must be given
+ * (the host programs apply vp2vs / vp2densityGardner, L/rockmod/modparconvert.c,
+ * when the files are absent).  On FDMOD_EUNSTABLE *dt_max holds the limit. */
+int fdmod_set_model(fdmod_ctx *ctx, const float *vp, const float *vs, const float *rho, const float *eps,
+                    const float *delta, const float *theta, double *dt_max);
+
+/* stype 1..8 (BT:41-56); positions in cells (0..nx / 0..nz inclusive, obsinfo.c:69-80);
+ * src is [ntrace][nt] (Source2D.src). */
+int fdmod_set_source(fdmod_ctx *ctx, int stype, int ntrace, const int *posx, const int *posz, const float *src);
+
+/* seistype 0..3 (BT:57-61); ntsample = (nt-1)/mt + 1 (obsinfo.c:199). */
+int fdmod_set_receivers(fdmod_ctx *ctx, int seistype, int mt, int ntrace, const int *posx, const int *posz);
+
+/* wsave 0..3, savew 0/1, snap_sample >= 1 (BT:79-92); fn may be NULL when wsave == 0. */
+int fdmod_set_snapshots(fdmod_ctx *ctx, int wsave, int savew, int snap_sample, fdmod_snapshot_fn fn, void *user);
+
+int fdmod_set_energy(fdmod_ctx *ctx, int on);
+
+/* Run loop iterations k0 .. k0+count-1 (it = k or nt-1-k).  The wavefield persists
+ * between calls, so a run may be split.  *seconds (may be NULL) receives the device
+ * time of the call, measured with HIP events around the loop. */
+int fdmod_run(fdmod_ctx *ctx, int k0, int count, double *seconds);
+
+/* Reset the wavefield, traces and energy to zero (model and geometry are kept). */
+int fdmod_reset(fdmod_ctx *ctx);
+
+/* Traces in Receiver2D.recv layout [ntrace][ntsample]. */
+int fdmod_get_traces(fdmod_ctx *ctx, int component, float *out);
+
+/* Energy series E, Ek, Ep of length nt (BT:946-950). */
+int fdmod_get_energy(fdmod_ctx *ctx, float *E, float *Ek, float *Ep);
+
+/* Padded arrays (Nx*Nz floats, iz fastest): model after MEAL, parameters, fields. */
+int fdmod_get_array(fdmod_ctx *ctx, int which, float *out);
+int fdmod_set_array(fdmod_ctx *ctx, int which, const float *in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDMOD_HIP_H */

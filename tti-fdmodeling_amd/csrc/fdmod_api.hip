@@ -1,0 +1,606 @@
+// libfdmod_hip C-ABI: context management, host-side validation, setup sequence
+// and the time-loop driver.  Declarations and the reference code each entry
+// point replaces are in include/fdmod_hip.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "fdmod_internal.h"
+
+using namespace fdm;
+
+struct fdmod_ctx {
+  fdmod_params p;
+  Grid g;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevArrays d;
+  float *pool = nullptr;
+  uint8_t *flags = nullptr;
+  bool model_set = false;
+  // source
+  int stype = 0, ns = 0, npass = 0, ntap = 0, seq = 1;
+  int *d_isx = nullptr, *d_isz = nullptr;
+  float *d_src = nullptr;
+  SrcTap *d_taps = nullptr;
+  // receivers
+  int seistype = 0, mt = 1, nr = 0, ntsample = 0;
+  int *d_gx = nullptr, *d_gz = nullptr;
+  float *d_rvx = nullptr, *d_rvz = nullptr, *d_rp = nullptr;
+  // snapshots
+  int wsave = 0, savew = 0, snap_sample = 1;
+  fdmod_snapshot_fn snap_fn = nullptr;
+  void *snap_user = nullptr;
+  float *d_p = nullptr;
+  std::vector<float> h_snap;
+  // energy
+  int energy = 0, eblocks = 0;
+  float *d_E = nullptr, *d_Ek = nullptr, *d_Ep = nullptr, *d_partial = nullptr;
+  std::string err;
+};
+
+static int fail(fdmod_ctx *c, int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCALL(c, x)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) return fail((c), FDMOD_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+static void dfree(void *p)
+{
+  if (p) (void)hipFree(p);
+}
+
+// which padded arrays a form needs
+static bool form_uses(int form, int a)
+{
+  switch (a) {
+  case FDMOD_A_C15: case FDMOD_A_C35: return form == FDMOD_FORM_BOND;
+  case FDMOD_A_SIN2: case FDMOD_A_COS2: case FDMOD_A_SIN2T:
+  case FDMOD_A_TXX: case FDMOD_A_TZZ: case FDMOD_A_TXZ: return form == FDMOD_FORM_TENSOR;
+  case FDMOD_A_SIN: case FDMOD_A_COS: case FDMOD_A_SINV: case FDMOD_A_COSV: return form == FDMOD_FORM_COORD;
+  default: return true;
+  }
+}
+
+extern "C" const char *fdmod_strerror(int code)
+{
+  switch (code) {
+  case FDMOD_OK: return "ok";
+  case FDMOD_EINVAL: return "invalid argument";
+  case FDMOD_EUNSTABLE: return "finite-difference scheme unstable for this dt";
+  case FDMOD_ERANGE: return "position outside the model";
+  case FDMOD_EDEVICE: return "HIP runtime error";
+  case FDMOD_ENOMEM: return "out of memory";
+  case FDMOD_ESTATE: return "call out of order";
+  default: return "unknown error";
+  }
+}
+
+extern "C" const char *fdmod_last_error(const fdmod_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int fdmod_dims(const fdmod_ctx *c, int *o)
+{
+  if (!c || !o) return FDMOD_EINVAL;
+  o[0] = c->g.Nz; o[1] = c->g.Nx; o[2] = c->g.e; o[3] = c->g.izStart; o[4] = c->g.izEnd;
+  o[5] = c->g.ixStart; o[6] = c->g.ixEnd;
+  return FDMOD_OK;
+}
+
+static void release(fdmod_ctx *c)
+{
+  dfree(c->pool); dfree(c->flags);
+  dfree(c->d_isx); dfree(c->d_isz); dfree(c->d_src); dfree(c->d_taps);
+  dfree(c->d_gx); dfree(c->d_gz); dfree(c->d_rvx); dfree(c->d_rvz); dfree(c->d_rp);
+  dfree(c->d_p); dfree(c->d_E); dfree(c->d_Ek); dfree(c->d_Ep); dfree(c->d_partial);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" void fdmod_destroy(fdmod_ctx *c)
+{
+  if (!c) return;
+  release(c);
+  delete c;
+}
+
+extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
+{
+  if (!p || !out) return FDMOD_EINVAL;
+  *out = nullptr;
+  if (p->form < 0 || p->form > 2) return FDMOD_EINVAL;
+  if (p->nz < 1 || p->nx < 1 || p->nt < 1 || p->ealThick < 0) return FDMOD_EINVAL;
+  if (!(p->dx > 0) || !(p->dz > 0) || !(p->dt > 0) || !(p->fpeak > 0)) return FDMOD_EINVAL;
+  if (p->freesurface != 0 && p->freesurface != 1) return FDMOD_EINVAL;
+  if (p->mode != 1 && p->mode != 2) return FDMOD_EINVAL;
+  fdmod_ctx *c = new fdmod_ctx();
+  c->p = *p;
+  Grid &g = c->g;
+  g.e = p->ealThick + kN;  // BT:307-308
+  g.Nz = p->nz + 2 * g.e;
+  g.Nx = p->nx + 2 * g.e;
+  g.ldz = (g.Nz + 63) / 64 * 64;
+  g.ixStart = kN;
+  g.ixEnd = g.Nx - kN;
+  g.izStart = p->freesurface == 0 ? kN : g.e;
+  g.izEnd = g.Nz - kN;
+  g.tilesZ = (g.izEnd - g.izStart + TZ - 1) / TZ;
+  g.tilesX = (g.ixEnd - g.ixStart + TX - 1) / TX;
+  g.invdx = (float)(1.0 / p->dx);
+  g.invdz = (float)(1.0 / p->dz);
+  g.dt = p->dt;
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+    fdmod_destroy(c);
+    return FDMOD_EDEVICE;
+  }
+  int nused = 0;
+  for (int a = 0; a < FDMOD_A_COUNT; a++) nused += form_uses(p->form, a);
+  size_t per = (size_t)g.Nx * g.ldz;
+  if (hipMalloc(&c->pool, per * nused * sizeof(float)) != hipSuccess) {
+    fdmod_destroy(c);
+    return FDMOD_ENOMEM;
+  }
+  int k = 0;
+  for (int a = 0; a < FDMOD_A_COUNT; a++) c->d.a[a] = form_uses(p->form, a) ? c->pool + per * (k++) : nullptr;
+  if (hipMemsetAsync(c->pool, 0, per * nused * sizeof(float), c->stream) != hipSuccess ||
+      hipMalloc(&c->flags, (size_t)g.tilesZ * g.tilesX + 1) != hipSuccess) {
+    fdmod_destroy(c);
+    return FDMOD_ENOMEM;
+  }
+  *out = c;
+  return FDMOD_OK;
+}
+
+// staggered_grid_stable2d (L/fdmod/stable.c:217-242) on the padded pre-MEAL vp;
+// the padded maximum equals the maximum of the unpadded input.
+static double stability_limit(const fdmod_ctx *c, const float *vp)
+{
+  static const float C[5] = {kC1, kC2, kC3, kC4, kC5};
+  double vp_max = __FLT_MIN__;
+  size_t n = (size_t)c->p.nz * c->p.nx;
+  for (size_t i = 0; i < n; i++) vp_max = fmaxf(vp[i], (float)vp_max);
+  double C_sum = 0.0f;
+  for (int i = 0; i < kN; i += 2) C_sum += C[i] * pow(-1.0, 1.0 * i);
+  float dx = c->p.dx, dz = c->p.dz;
+  return 1.0f / (vp_max * sqrt((1.0 / (dx * dx) + 1.0 / (dz * dz)) * C_sum));
+}
+
+extern "C" int fdmod_set_model(fdmod_ctx *c, const float *vp, const float *vs, const float *rho, const float *eps,
+                               const float *delta, const float *theta, double *dt_max)
+{
+  if (!c) return FDMOD_EINVAL;
+  if (!vp || !vs || !rho || !eps || !delta || !theta) return fail(c, FDMOD_EINVAL, "model array is NULL");
+  const fdmod_params &p = c->p;
+  Grid &g = c->g;
+  double lim = stability_limit(c, vp);
+  if (dt_max) *dt_max = lim;
+  if (p.dt > lim)  // stable.c:236-241
+    return fail(c, FDMOD_EUNSTABLE,
+                "the finite differenct scheme is unstable! the time step size must smaller than %5.5f!", lim);
+  size_t n = (size_t)p.nz * p.nx;
+  float *stage = nullptr;
+  HIPCALL(c, hipMalloc(&stage, 6 * n * sizeof(float) + 256 * sizeof(float)));
+  const float *in[6] = {vp, vs, rho, eps, delta, theta};
+  for (int k = 0; k < 6; k++) {
+    hipError_t e = hipMemcpyAsync(stage + k * n, in[k], n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+      dfree(stage);
+      return fail(c, FDMOD_EDEVICE, "model upload: %s", hipGetErrorString(e));
+    }
+    launch_pad(c->stream, stage + k * n, c->d.a[FDMOD_A_VP + k], p.nz, p.nx, g);
+  }
+  // MEAL constants on the host, in the reference's float order (meal.c:63-77, 23-29)
+  int nlayer = g.e;
+  float alpha = p.eal_alpha == -1.0f ? 2.5f : p.eal_alpha;
+  float pd = 2;
+  float Rcoef = powf(10.0f, -1.0f * ((log10f((float)nlayer) - 1.0f) / (log10f(2.0f))) - 3.0f - 0.5f * pd);
+  std::vector<float> wtab(nlayer + 1, 0.0f);
+  for (int l = 1; l <= nlayer; l++) {
+    float d = (0.1f * l) / (0.1f * nlayer);
+    wtab[l] = expf(logf(2.0f) * powf(d, alpha)) - 1.0f;
+  }
+  float *d_wtab = stage + 6 * n;
+  int rc = FDMOD_OK;
+  if (nlayer + 1 > 256) {
+    dfree(stage);
+    return fail(c, FDMOD_EINVAL, "ealThick too large (%d)", p.ealThick);
+  }
+  if (hipMemcpyAsync(d_wtab, wtab.data(), wtab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream) !=
+      hipSuccess)
+    rc = FDMOD_EDEVICE;
+  float k0 = -1.0f * (1.0f + pd);
+  float thx = nlayer * p.dx, thz = nlayer * p.dz;
+  if (rc == FDMOD_OK) {
+    launch_meal(c->stream, g, c->d.a[FDMOD_A_VP], c->d.a[FDMOD_A_VS], c->d.a[FDMOD_A_RHO], c->d.a[FDMOD_A_D],
+                c->d.a[FDMOD_A_BETA], c->d.a[FDMOD_A_COEF1], c->d.a[FDMOD_A_COEF2], d_wtab, nlayer, p.freesurface,
+                k0, logf(Rcoef), 2.0f * thx, 2.0f * thz, p.dx * p.fpeak, p.dz * p.fpeak, sqrtf(3.0f), (double)p.dt);
+    launch_cij(c->stream, g, p.form, c->d, p.scalar);
+    launch_stagger(c->stream, g, p.form, c->d);
+    if (p.form == FDMOD_FORM_BOND && p.freesurface == 1) launch_fs_fixup(c->stream, g, c->d);
+    launch_tile_flags(c->stream, g, c->d.a[FDMOD_A_COEF1], c->d.a[FDMOD_A_COEF2], c->flags);
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  dfree(stage);
+  if (rc != FDMOD_OK || e != hipSuccess || (e = hipGetLastError()) != hipSuccess)
+    return fail(c, FDMOD_EDEVICE, "model setup: %s", hipGetErrorString(e));
+  c->model_set = true;
+  return fdmod_reset(c);
+}
+
+extern "C" int fdmod_reset(fdmod_ctx *c)
+{
+  if (!c) return FDMOD_EINVAL;
+  size_t per = (size_t)c->g.Nx * c->g.ldz * sizeof(float);
+  for (int a = FDMOD_A_VX; a <= FDMOD_A_TXZ; a++)
+    if (c->d.a[a]) HIPCALL(c, hipMemsetAsync(c->d.a[a], 0, per, c->stream));
+  size_t nrec = (size_t)c->nr * c->ntsample * sizeof(float);
+  if (c->d_rvx) HIPCALL(c, hipMemsetAsync(c->d_rvx, 0, nrec, c->stream));
+  if (c->d_rvz) HIPCALL(c, hipMemsetAsync(c->d_rvz, 0, nrec, c->stream));
+  if (c->d_rp) HIPCALL(c, hipMemsetAsync(c->d_rp, 0, nrec, c->stream));
+  if (c->d_E) {
+    HIPCALL(c, hipMemsetAsync(c->d_E, 0, c->p.nt * sizeof(float), c->stream));
+    HIPCALL(c, hipMemsetAsync(c->d_Ek, 0, c->p.nt * sizeof(float), c->stream));
+    HIPCALL(c, hipMemsetAsync(c->d_Ep, 0, c->p.nt * sizeof(float), c->stream));
+  }
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return FDMOD_OK;
+}
+
+// Source taps in the reference's float order (BT:706-834, 893-914):
+// alpha = logf(0.8)/(dx^2+dz^2), w = expf(alpha*(rx^2+rz^2))*invdx*invdz*dt.
+static void build_taps(const fdmod_params &p, int stype, std::vector<SrcTap> &taps, int &ntap, int &npass)
+{
+  float dx = p.dx, dz = p.dz, dt = p.dt;
+  float invdx = (float)(1.0 / dx), invdz = (float)(1.0 / dz);
+  float alpha = logf(0.8f) / (dx * dx + dz * dz);
+  taps.clear();
+  if (stype == 8) {  // BT:814-833
+    float q = 0.125f * sqrtf(2.0f);
+    taps = {{0, 0, -q, q}, {-1, 0, -q, -q}, {-1, -1, q, -q}, {0, -1, q, q}};
+    ntap = 4;
+    npass = 1;
+    return;
+  }
+  ntap = (2 * kN + 1) * (2 * kN + 1);
+  npass = stype == 4 ? 2 : 1;
+  for (int pass = 0; pass < npass; pass++) {
+    int kind = stype == 4 ? (pass == 0 ? 2 : 3) : stype;
+    for (int ox = -kN; ox <= kN; ox++) {
+      float rx = ox * dx;
+      for (int oz = -kN; oz <= kN; oz++) {
+        float rz = oz * dz;
+        float w = expf(alpha * (rx * rx + rz * rz)) * invdx * invdz * dt;
+        SrcTap t{ox, oz, 0.0f, 0.0f};
+        switch (kind) {
+        case 1: t.ca = 0.5f * w; t.cb = 0.5f * w; break;
+        case 2: t.ca = 2.0f * alpha * rx * w; t.cb = 2.0f * alpha * rz * w; break;
+        case 3: t.ca = -(2.0f * alpha * rz * w); t.cb = 2.0f * alpha * rx * w; break;
+        case 5: case 6: t.ca = w; break;
+        case 7: t.ca = 0.5f * sqrtf(2.0f) * w; t.cb = 0.5f * sqrtf(2.0f) * w; break;
+        }
+        taps.push_back(t);
+      }
+    }
+  }
+}
+
+extern "C" int fdmod_set_source(fdmod_ctx *c, int stype, int ns, const int *posx, const int *posz, const float *src)
+{
+  if (!c) return FDMOD_EINVAL;
+  if (stype < 1 || stype > 8) return fail(c, FDMOD_EINVAL, "stype must be 1..8 (got %d)", stype);
+  if (ns < 0 || (ns > 0 && (!posx || !posz || !src))) return fail(c, FDMOD_EINVAL, "bad source arrays");
+  const fdmod_params &p = c->p;
+  for (int i = 0; i < ns; i++) {  // obsinfo.c:69-80
+    if (posz[i] < 0 || posz[i] > p.nz)
+      return fail(c, FDMOD_ERANGE, "the position of source in 1st dimension (%d) must be between 0 and %d!", posz[i],
+                  p.nz);
+    if (posx[i] < 0 || posx[i] > p.nx)
+      return fail(c, FDMOD_ERANGE, "the position of source in 2nd dimension (%d) must be between 0 and %d!", posx[i],
+                  p.nx);
+  }
+  std::vector<SrcTap> taps;
+  build_taps(p, stype, taps, c->ntap, c->npass);
+  std::vector<int> isx(ns), isz(ns);
+  for (int i = 0; i < ns; i++) {
+    isx[i] = posx[i] + c->g.e;
+    isz[i] = posz[i] + c->g.e;
+  }
+  // sources may run in parallel only if no two footprints share a cell
+  int rad = stype == 8 ? 1 : kN;
+  std::vector<int> order(ns);
+  for (int i = 0; i < ns; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return isx[a] < isx[b]; });
+  bool overlap = false;
+  for (int a = 0; a < ns && !overlap; a++)
+    for (int b = a + 1; b < ns && isx[order[b]] - isx[order[a]] <= 2 * rad; b++)
+      if (abs(isz[order[b]] - isz[order[a]]) <= 2 * rad) {
+        overlap = true;
+        break;
+      }
+  c->seq = overlap ? 1 : 0;
+  dfree(c->d_isx); dfree(c->d_isz); dfree(c->d_src); dfree(c->d_taps);
+  c->d_isx = c->d_isz = nullptr; c->d_src = nullptr; c->d_taps = nullptr;
+  c->stype = stype;
+  c->ns = ns;
+  if (ns == 0) return FDMOD_OK;
+  HIPCALL(c, hipMalloc(&c->d_isx, ns * sizeof(int)));
+  HIPCALL(c, hipMalloc(&c->d_isz, ns * sizeof(int)));
+  HIPCALL(c, hipMalloc(&c->d_src, (size_t)ns * p.nt * sizeof(float)));
+  HIPCALL(c, hipMalloc(&c->d_taps, taps.size() * sizeof(SrcTap)));
+  HIPCALL(c, hipMemcpy(c->d_isx, isx.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+  HIPCALL(c, hipMemcpy(c->d_isz, isz.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+  HIPCALL(c, hipMemcpy(c->d_src, src, (size_t)ns * p.nt * sizeof(float), hipMemcpyHostToDevice));
+  HIPCALL(c, hipMemcpy(c->d_taps, taps.data(), taps.size() * sizeof(SrcTap), hipMemcpyHostToDevice));
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, const int *posx, const int *posz)
+{
+  if (!c) return FDMOD_EINVAL;
+  if (seistype < 0 || seistype > 3) return fail(c, FDMOD_EINVAL, "seistype must be 0..3 (got %d)", seistype);
+  if (mt < 1) return fail(c, FDMOD_EINVAL, "mt must be >= 1");
+  if (nr < 0 || (nr > 0 && (!posx || !posz))) return fail(c, FDMOD_EINVAL, "bad receiver arrays");
+  const fdmod_params &p = c->p;
+  for (int i = 0; i < nr; i++) {  // obsinfo.c:206-217
+    if (posz[i] < 0 || posz[i] > p.nz)
+      return fail(c, FDMOD_ERANGE, "the position of receiver in 1st dimension (%d) must be between 0 and %d!",
+                  posz[i], p.nz);
+    if (posx[i] < 0 || posx[i] > p.nx)
+      return fail(c, FDMOD_ERANGE, "the position of receiver in 2nd dimension (%d) must be between 0 and %d!",
+                  posx[i], p.nx);
+  }
+  dfree(c->d_gx); dfree(c->d_gz); dfree(c->d_rvx); dfree(c->d_rvz); dfree(c->d_rp);
+  c->d_gx = c->d_gz = nullptr; c->d_rvx = c->d_rvz = c->d_rp = nullptr;
+  c->seistype = seistype;
+  c->mt = mt;
+  c->nr = nr;
+  c->ntsample = (p.nt - 1) / mt + 1;  // obsinfo.c:199
+  if (nr == 0 || seistype == 0) return FDMOD_OK;
+  std::vector<int> gx(nr), gz(nr);
+  for (int i = 0; i < nr; i++) {
+    gx[i] = posx[i] + c->g.e;
+    gz[i] = posz[i] + c->g.e;
+  }
+  size_t nrec = (size_t)nr * c->ntsample * sizeof(float);
+  HIPCALL(c, hipMalloc(&c->d_gx, nr * sizeof(int)));
+  HIPCALL(c, hipMalloc(&c->d_gz, nr * sizeof(int)));
+  HIPCALL(c, hipMemcpy(c->d_gx, gx.data(), nr * sizeof(int), hipMemcpyHostToDevice));
+  HIPCALL(c, hipMemcpy(c->d_gz, gz.data(), nr * sizeof(int), hipMemcpyHostToDevice));
+  if (seistype == 1 || seistype == 3) {
+    HIPCALL(c, hipMalloc(&c->d_rvx, nrec));
+    HIPCALL(c, hipMalloc(&c->d_rvz, nrec));
+    HIPCALL(c, hipMemsetAsync(c->d_rvx, 0, nrec, c->stream));
+    HIPCALL(c, hipMemsetAsync(c->d_rvz, 0, nrec, c->stream));
+  }
+  if (seistype == 2 || seistype == 3) {
+    HIPCALL(c, hipMalloc(&c->d_rp, nrec));
+    HIPCALL(c, hipMemsetAsync(c->d_rp, 0, nrec, c->stream));
+  }
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_set_snapshots(fdmod_ctx *c, int wsave, int savew, int snap_sample, fdmod_snapshot_fn fn,
+                                   void *user)
+{
+  if (!c) return FDMOD_EINVAL;
+  if (wsave < 0 || wsave > 5 || (savew != 0 && savew != 1)) return fail(c, FDMOD_EINVAL, "bad wsave/savew");
+  if (wsave > 0 && (snap_sample < 1 || !fn)) return fail(c, FDMOD_EINVAL, "snapshots need snap_sample>=1 and a sink");
+  c->wsave = wsave;
+  c->savew = savew;
+  c->snap_sample = snap_sample < 1 ? 1 : snap_sample;
+  c->snap_fn = fn;
+  c->snap_user = user;
+  if (wsave >= 2 && !c->d_p) HIPCALL(c, hipMalloc(&c->d_p, (size_t)c->g.Nx * c->g.ldz * sizeof(float)));
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_set_energy(fdmod_ctx *c, int on)
+{
+  if (!c) return FDMOD_EINVAL;
+  c->energy = on ? 1 : 0;
+  if (c->energy && !c->d_E) {
+    c->eblocks = 512;
+    HIPCALL(c, hipMalloc(&c->d_E, c->p.nt * sizeof(float)));
+    HIPCALL(c, hipMalloc(&c->d_Ek, c->p.nt * sizeof(float)));
+    HIPCALL(c, hipMalloc(&c->d_Ep, c->p.nt * sizeof(float)));
+    HIPCALL(c, hipMalloc(&c->d_partial, 2 * c->eblocks * sizeof(float)));
+    HIPCALL(c, hipMemsetAsync(c->d_E, 0, c->p.nt * sizeof(float), c->stream));
+    HIPCALL(c, hipMemsetAsync(c->d_Ek, 0, c->p.nt * sizeof(float), c->stream));
+    HIPCALL(c, hipMemsetAsync(c->d_Ep, 0, c->p.nt * sizeof(float), c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+  }
+  return FDMOD_OK;
+}
+
+static SweepArgs sweep_args(fdmod_ctx *c)
+{
+  SweepArgs a;
+  float *const *d = c->d.a;
+  a.g = c->g;
+  a.vx = d[FDMOD_A_VX]; a.vz = d[FDMOD_A_VZ];
+  a.sxx = d[FDMOD_A_SXX]; a.szz = d[FDMOD_A_SZZ]; a.sxz = d[FDMOD_A_SXZ];
+  a.txx = d[FDMOD_A_TXX]; a.tzz = d[FDMOD_A_TZZ]; a.txz = d[FDMOD_A_TXZ];
+  a.coef1 = d[FDMOD_A_COEF1]; a.coef2 = d[FDMOD_A_COEF2]; a.invrho = d[FDMOD_A_INVRHO];
+  a.c11 = d[FDMOD_A_C11]; a.c13 = d[FDMOD_A_C13]; a.c15 = d[FDMOD_A_C15];
+  a.c33 = d[FDMOD_A_C33]; a.c35 = d[FDMOD_A_C35]; a.c55 = d[FDMOD_A_C55];
+  a.sin2 = d[FDMOD_A_SIN2]; a.cos2 = d[FDMOD_A_COS2]; a.sin2t = d[FDMOD_A_SIN2T];
+  a.sn = d[FDMOD_A_SIN]; a.cs = d[FDMOD_A_COS]; a.snv = d[FDMOD_A_SINV]; a.csv = d[FDMOD_A_COSV];
+  a.undamped = c->flags;
+  a.freesurface = c->p.freesurface;
+  return a;
+}
+
+static int snapshot(fdmod_ctx *c, int it)
+{
+  const Grid &g = c->g;
+  int e = c->savew == 0 ? g.e : 0;
+  int n1 = g.Nz - 2 * e, n2 = g.Nx - 2 * e;
+  c->h_snap.resize((size_t)n1 * n2);
+  auto emit = [&](const float *dev, const char *kind) -> int {
+    HIPCALL(c, hipMemcpy2DAsync(c->h_snap.data(), n1 * sizeof(float), dev + (size_t)e * g.ldz + e,
+                                g.ldz * sizeof(float), n1 * sizeof(float), n2, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    if (c->snap_fn(c->snap_user, it, kind, c->h_snap.data(), n1, n2) != 0)
+      return fail(c, FDMOD_EINVAL, "snapshot sink failed at it=%d", it);
+    return FDMOD_OK;
+  };
+  int rc = FDMOD_OK;
+  int ws = c->wsave;
+  // isoelastic.c:249-288: 1 = vx,vz; 2 = p; 3 = both; 4/5 = vx,vz under P/S names
+  if (ws == 1 || ws == 3) {
+    if ((rc = emit(c->d.a[FDMOD_A_VX], "vx"))) return rc;
+    if ((rc = emit(c->d.a[FDMOD_A_VZ], "vz"))) return rc;
+  }
+  if (ws == 2 || ws == 3) {
+    launch_pressure(c->stream, c->d.a[FDMOD_A_SXX], c->d.a[FDMOD_A_SZZ], c->d_p, g.Nx, g.ldz);
+    if ((rc = emit(c->d_p, "p"))) return rc;
+  }
+  if (ws == 4 || ws == 5) {
+    if ((rc = emit(c->d.a[FDMOD_A_VX], ws == 4 ? "vxp" : "vxs"))) return rc;
+    if ((rc = emit(c->d.a[FDMOD_A_VZ], ws == 4 ? "vzp" : "vzs"))) return rc;
+  }
+  return rc;
+}
+
+// One iteration of the reference loop (BT:651-954): V sweep, velocity sources,
+// S sweep (+ free surface), stress source, snapshot, receivers, energy.
+static int step(fdmod_ctx *c, int it, const SweepArgs &sa)
+{
+  const fdmod_params &p = c->p;
+  launch_vsweep(c->stream, p.form, sa);
+  SourceArgs src;
+  if (c->ns > 0) {
+    src.g = c->g;
+    src.isx = c->d_isx; src.isz = c->d_isz; src.src = c->d_src;
+    src.nt = p.nt; src.ns = c->ns; src.it = it;
+    src.taps = c->d_taps; src.ntap = c->ntap; src.npass = c->npass; src.sequential = c->seq;
+    src.fc = src.fd = nullptr;
+  }
+  if (c->ns > 0 && c->stype != 1) {
+    src.fa = c->stype == 6 ? sa.vz : sa.vx;
+    src.fb = (c->stype == 5 || c->stype == 6) ? nullptr : sa.vz;
+    launch_source(c->stream, src);
+  }
+  launch_ssweep(c->stream, p.form, sa);
+  if (c->ns > 0 && c->stype == 1) {
+    src.fa = sa.sxx;
+    src.fb = sa.szz;
+    if (p.form == FDMOD_FORM_TENSOR) {  // TR:888-891
+      src.fc = sa.txx;
+      src.fd = sa.tzz;
+    }
+    launch_source(c->stream, src);
+  }
+  if (c->wsave > 0 && it % c->snap_sample == 0) {
+    int rc = snapshot(c, it);
+    if (rc) return rc;
+  }
+  if (c->seistype > 0 && c->nr > 0 && it % c->mt == 0) {
+    RecvArgs r;
+    r.nr = c->nr; r.ldz = c->g.ldz; r.gx = c->d_gx; r.gz = c->d_gz;
+    r.vx = sa.vx; r.vz = sa.vz; r.sxx = sa.sxx; r.szz = sa.szz;
+    r.out_vx = c->d_rvx; r.out_vz = c->d_rvz; r.out_p = c->d_rp;
+    r.isample = it / c->mt;
+    r.want_v = c->seistype == 1 || c->seistype == 3;
+    r.want_p = c->seistype == 2 || c->seistype == 3;
+    launch_receivers(c->stream, r);
+  }
+  if (c->energy) {
+    launch_energy_partial(c->stream, c->g, p.nx, p.nz, c->d, c->d_partial, c->eblocks);
+    launch_energy_final(c->stream, c->d_partial, c->eblocks, p.dx, p.dz, c->d_E, c->d_Ek, c->d_Ep, it);
+  }
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_run(fdmod_ctx *c, int k0, int count, double *seconds)
+{
+  if (!c) return FDMOD_EINVAL;
+  if (!c->model_set) return fail(c, FDMOD_ESTATE, "fdmod_run before fdmod_set_model");
+  if (k0 < 0 || count < 0) return fail(c, FDMOD_EINVAL, "bad iteration range");
+  const fdmod_params &p = c->p;
+  int k1 = std::min(k0 + count, p.nt);
+  SweepArgs sa = sweep_args(c);
+  HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+  for (int k = k0; k < k1; k++) {
+    int it = p.mode == 1 ? k : p.nt - 1 - k;
+    int rc = step(c, it, sa);
+    if (rc) return rc;
+  }
+  HIPCALL(c, hipEventRecord(c->ev1, c->stream));
+  HIPCALL(c, hipEventSynchronize(c->ev1));
+  HIPCALL(c, hipGetLastError());
+  float ms = 0.0f;
+  HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (seconds) *seconds = ms * 1e-3;
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_get_traces(fdmod_ctx *c, int comp, float *out)
+{
+  if (!c || !out) return FDMOD_EINVAL;
+  float *d = comp == FDMOD_TRACE_VX ? c->d_rvx : comp == FDMOD_TRACE_VZ ? c->d_rvz : comp == FDMOD_TRACE_P ? c->d_rp
+                                                                                                           : nullptr;
+  if (!d) return fail(c, FDMOD_EINVAL, "component %d was not recorded", comp);
+  std::vector<float> tmp((size_t)c->nr * c->ntsample);
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  HIPCALL(c, hipMemcpy(tmp.data(), d, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int s = 0; s < c->ntsample; s++)
+    for (int r = 0; r < c->nr; r++) out[(size_t)r * c->ntsample + s] = tmp[(size_t)s * c->nr + r];
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_get_energy(fdmod_ctx *c, float *E, float *Ek, float *Ep)
+{
+  if (!c || !E || !Ek || !Ep) return FDMOD_EINVAL;
+  if (!c->d_E) return fail(c, FDMOD_ESTATE, "energy was not enabled");
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  HIPCALL(c, hipMemcpy(E, c->d_E, c->p.nt * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCALL(c, hipMemcpy(Ek, c->d_Ek, c->p.nt * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCALL(c, hipMemcpy(Ep, c->d_Ep, c->p.nt * sizeof(float), hipMemcpyDeviceToHost));
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_get_array(fdmod_ctx *c, int which, float *out)
+{
+  if (!c || !out || which < 0 || which >= FDMOD_A_COUNT) return FDMOD_EINVAL;
+  if (!c->d.a[which]) return fail(c, FDMOD_EINVAL, "array %d is not used by this form", which);
+  const Grid &g = c->g;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  HIPCALL(c, hipMemcpy2D(out, g.Nz * sizeof(float), c->d.a[which], g.ldz * sizeof(float), g.Nz * sizeof(float), g.Nx,
+                         hipMemcpyDeviceToHost));
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_set_array(fdmod_ctx *c, int which, const float *in)
+{
+  if (!c || !in || which < 0 || which >= FDMOD_A_COUNT) return FDMOD_EINVAL;
+  if (!c->d.a[which]) return fail(c, FDMOD_EINVAL, "array %d is not used by this form", which);
+  const Grid &g = c->g;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  HIPCALL(c, hipMemcpy2D(c->d.a[which], g.ldz * sizeof(float), in, g.Nz * sizeof(float), g.Nz * sizeof(float), g.Nx,
+                         hipMemcpyHostToDevice));
+  if (which == FDMOD_A_COEF1 || which == FDMOD_A_COEF2) {
+    launch_tile_flags(c->stream, g, c->d.a[FDMOD_A_COEF1], c->d.a[FDMOD_A_COEF2], c->flags);
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+  }
+  return FDMOD_OK;
+}

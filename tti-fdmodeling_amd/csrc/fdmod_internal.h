@@ -1,0 +1,107 @@
+// Internal definitions shared by the libfdmod_hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/fdmod_hip.h"
+
+namespace fdm {
+
+// Radius of the RSG stencil (N in the reference, BT:307).
+constexpr int kN = 5;
+
+// Least-squares RSG coefficients (BT:464-465).
+constexpr float kC1 = 1.24362739770907f;
+constexpr float kC2 = -0.114021164233934f;
+constexpr float kC3 = 0.0272965993810630f;
+constexpr float kC4 = -0.00699199493927320f;
+constexpr float kC5 = 0.00132022981661381f;
+
+// PI as defined by Seismic Unix cwp.h, used by the theta conversion (BT:508).
+constexpr double kPI = 3.141592653589793;
+
+// Sweep tile: TZ rows (one per lane of a wave) by TX columns; 4 waves per block,
+// each wave owns TX/4 columns.  Both sweeps use the same tiling so that one
+// per-tile "undamped" flag serves both.
+constexpr int TZ = 64;
+constexpr int TX = 32;
+constexpr int kThreads = 256;
+constexpr int kColsPerWave = TX / (kThreads / 64);
+
+// Device-side geometry of a padded grid.
+struct Grid {
+  int Nz, Nx, ldz;         // padded sizes; ldz = column pitch in floats (>= Nz)
+  int ixStart, ixEnd, izStart, izEnd;
+  int e;                   // effective ealThick (user value + kN)
+  int tilesZ, tilesX;      // sweep tiles over the update region
+  float invdx, invdz, dt;  // float(1.0/dx), float(1.0/dz), dt
+};
+
+// Device pointers of one modelling context (all padded Nx*ldz floats).
+struct DevArrays {
+  float *a[FDMOD_A_COUNT];
+};
+
+struct SweepArgs {
+  Grid g;
+  // fields
+  float *vx, *vz, *sxx, *szz, *sxz, *txx, *tzz, *txz;
+  // parameters
+  const float *coef1, *coef2, *invrho;
+  const float *c11, *c13, *c15, *c33, *c35, *c55;
+  const float *sin2, *cos2, *sin2t, *sn, *cs, *snv, *csv;
+  const uint8_t *undamped;  // per tile: coef1 == 1 and coef2 == dt everywhere in the tile
+  int freesurface;
+};
+
+// Source injection table entry: offset from the source cell and the two
+// coefficients that multiply src[it] (already including the reference's
+// weight products, computed on the host in the reference's float order).
+struct SrcTap {
+  int ox, oz;
+  float ca, cb;
+};
+
+struct SourceArgs {
+  Grid g;
+  float *fa, *fb;           // targets: vx/vz, or sxx/szz
+  float *fc, *fd;           // tensor form stress source also adds to txx/tzz (else null)
+  const int *isx, *isz;     // padded source cells, per source
+  const float *src;         // [ns][nt]
+  int nt, ns, it;
+  const SrcTap *taps;       // [npass][ntap]
+  int ntap, npass;
+  int sequential;           // 1: one block walks the sources in order
+};
+
+struct RecvArgs {
+  int nr, ldz;
+  const int *gx, *gz;       // padded receiver cells
+  const float *vx, *vz, *sxx, *szz;
+  float *out_vx, *out_vz, *out_p;  // device [ntsample][nr]
+  int isample, want_v, want_p;
+};
+
+// ---- launchers (kernels_setup.hip) ----
+void launch_pad(hipStream_t s, const float *raw, float *dst, int nz, int nx, const Grid &g);
+void launch_meal(hipStream_t s, const Grid &g, float *vp, float *vs, float *rho, float *D, float *Beta,
+                 float *coef1, float *coef2, const float *wtab, int nlayer, int freesurface, float k0, float lnR,
+                 float twoThx, float twoThz, float dxf, float dzf, float sqrt3, double dt);
+void launch_cij(hipStream_t s, const Grid &g, int form, const DevArrays &d, float scalar);
+void launch_stagger(hipStream_t s, const Grid &g, int form, const DevArrays &d);
+void launch_fs_fixup(hipStream_t s, const Grid &g, const DevArrays &d);
+void launch_tile_flags(hipStream_t s, const Grid &g, const float *coef1, const float *coef2, uint8_t *flags);
+
+// ---- launchers (kernels_step.hip) ----
+void launch_vsweep(hipStream_t s, int form, const SweepArgs &a);
+void launch_ssweep(hipStream_t s, int form, const SweepArgs &a);
+void launch_source(hipStream_t s, const SourceArgs &a);
+void launch_receivers(hipStream_t s, const RecvArgs &a);
+void launch_pressure(hipStream_t s, const float *sxx, const float *szz, float *p, int Nx, int ldz);
+void launch_energy_partial(hipStream_t s, const Grid &g, int nx, int nz, const DevArrays &d, float *partial,
+                           int nblocks);
+void launch_energy_final(hipStream_t s, const float *partial, int nblocks, float dx, float dz, float *E, float *Ek,
+                         float *Ep, int it);
+
+}  // namespace fdm

@@ -1,0 +1,128 @@
+"""Python front end of libfdmod_hip: one modelling context per object.
+
+Mirrors the per-shot flow of the reference programs
+(src/main/sufdmod2dTTIelastic/sufdmod2dTTIelastic_*.c): create the padded grid,
+set the model, the source and the receivers, run the time loop, read traces.
+All work goes through the C-ABI in include/fdmod_hip.h.
+"""
+import ctypes as ct
+
+import numpy as np
+
+from ._lib import AID, FORMS, FdmodError, Params, SNAPSHOT_FN, _f32, _ptr, load
+
+__all__ = ["Modeling", "FdmodError", "FORMS", "load"]
+
+
+class Modeling:
+    """A single-shot modelling context on the GPU.
+
+    Arrays follow the reference layout: model inputs are (nx, nz) float32 with iz
+    fastest; padded arrays returned by get() are (Nx, Nz); traces are
+    (ntrace, ntsample) like Receiver2D.recv.
+    """
+
+    def __init__(self, form, nz, nx, nt, dx, dz, dt, ealThick=30, eal_alpha=-1.0, freesurface=0,
+                 fpeak=10.0, scalar=1.0, mode=1):
+        self.L = load()
+        p = Params(FORMS[form] if isinstance(form, str) else int(form), nz, nx, nt, dx, dz, dt, ealThick,
+                   eal_alpha, freesurface, fpeak, scalar, mode)
+        h = ct.c_void_p()
+        rc = self.L.fdmod_create(ct.byref(p), ct.byref(h))
+        if rc != 0:
+            raise FdmodError(rc, self.L.fdmod_strerror(rc).decode())
+        self.c = h
+        self.form, self.nz, self.nx, self.nt = form, nz, nx, nt
+        d = (ct.c_int * 7)()
+        self.L.fdmod_dims(self.c, d)
+        self.Nz, self.Nx, self.e, self.izStart, self.izEnd, self.ixStart, self.ixEnd = list(d)
+        self.nr = 0
+        self.ntsample = 0
+        self._snap_cb = None
+
+    def close(self):
+        if getattr(self, "c", None):
+            self.L.fdmod_destroy(self.c)
+            self.c = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FdmodError(rc, self.L.fdmod_last_error(self.c).decode())
+
+    def set_model(self, vp, vs, rho, eps, delta, theta):
+        arrs = [_f32(a).reshape(-1) for a in (vp, vs, rho, eps, delta, theta)]
+        for a in arrs:
+            if a.size != self.nz * self.nx:
+                raise ValueError("model arrays must have nz*nx samples")
+        dtm = ct.c_double(0)
+        rc = self.L.fdmod_set_model(self.c, *[_ptr(a, ct.c_float) for a in arrs], ct.byref(dtm))
+        self.dt_max = dtm.value
+        self._check(rc)
+
+    def set_source(self, stype, posx, posz, src):
+        posx = np.ascontiguousarray(posx, dtype=np.int32)
+        posz = np.ascontiguousarray(posz, dtype=np.int32)
+        src = _f32(src).reshape(len(posx), self.nt)
+        self._check(self.L.fdmod_set_source(self.c, stype, len(posx), _ptr(posx, ct.c_int), _ptr(posz, ct.c_int),
+                                            _ptr(src, ct.c_float)))
+
+    def set_receivers(self, seistype, mt, posx, posz):
+        posx = np.ascontiguousarray(posx, dtype=np.int32)
+        posz = np.ascontiguousarray(posz, dtype=np.int32)
+        self._check(self.L.fdmod_set_receivers(self.c, seistype, mt, len(posx), _ptr(posx, ct.c_int),
+                                               _ptr(posz, ct.c_int)))
+        self.nr = len(posx)
+        self.ntsample = (self.nt - 1) // mt + 1
+
+    def set_snapshots(self, wsave, savew, snap_sample, sink):
+        """sink(it, kind, array[n2, n1]) is called at every snapshot time."""
+
+        def cb(user, it, kind, data, n1, n2):
+            arr = np.ctypeslib.as_array(data, shape=(n2 * n1,)).reshape(n2, n1).copy()
+            try:
+                sink(it, kind.decode(), arr)
+                return 0
+            except Exception:  # noqa: BLE001 - reported to the library as a failed sink
+                return 1
+
+        self._snap_cb = SNAPSHOT_FN(cb)
+        self._check(self.L.fdmod_set_snapshots(self.c, wsave, savew, snap_sample, self._snap_cb, None))
+
+    def set_energy(self, on=1):
+        self._check(self.L.fdmod_set_energy(self.c, on))
+
+    def run(self, k0=0, count=None):
+        """Run loop iterations k0 .. k0+count-1; returns device seconds."""
+        if count is None:
+            count = self.nt - k0
+        sec = ct.c_double(0)
+        self._check(self.L.fdmod_run(self.c, k0, count, ct.byref(sec)))
+        return sec.value
+
+    def reset(self):
+        self._check(self.L.fdmod_reset(self.c))
+
+    def traces(self, comp):
+        """comp: 1 = vx, 3 = vz, 4 = p (traces2d_output flags)."""
+        out = np.empty(self.nr * self.ntsample, dtype=np.float32)
+        self._check(self.L.fdmod_get_traces(self.c, comp, _ptr(out, ct.c_float)))
+        return out.reshape(self.nr, self.ntsample)
+
+    def energy(self):
+        E, Ek, Ep = (np.empty(self.nt, dtype=np.float32) for _ in range(3))
+        self._check(self.L.fdmod_get_energy(self.c, _ptr(E, ct.c_float), _ptr(Ek, ct.c_float), _ptr(Ep, ct.c_float)))
+        return E, Ek, Ep
+
+    def get(self, name):
+        out = np.empty(self.Nx * self.Nz, dtype=np.float32)
+        self._check(self.L.fdmod_get_array(self.c, AID[name], _ptr(out, ct.c_float)))
+        return out.reshape(self.Nx, self.Nz)
+
+    def set(self, name, arr):
+        a = _f32(arr).reshape(-1)
+        if a.size != self.Nx * self.Nz:
+            raise ValueError("padded arrays must have Nx*Nz samples")
+        self._check(self.L.fdmod_set_array(self.c, AID[name], _ptr(a, ct.c_float)))
