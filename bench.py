@@ -1,0 +1,208 @@
+#!/usr/bin/env python
+"""Benchmark: plain forward RSG shot on one MI355X, 1126 x 7872 model, FP32.
+
+One "step" is one time step of the modelling loop (velocity sweep, stress sweep,
+free surface, source, receivers) over the full padded grid.  The workload is the
+full-size marine TTI model (nz=1126, nx=7872, dx=dz=10 m, dt=0.8 ms, ealThick=60,
+free surface on, one 12 Hz Ricker source, 7872 pressure receivers), built from a
+fixed seed; inputs are resident in HBM before the timed region.
+
+value = updated grid points per step x steps / wall time  (Gpts/s), where the
+updated points are (Nx-10) * (Nz-5-izStart) as in the reference loop bounds.
+
+Also reported:
+  roofline      the stress sweep (the larger of the two sweeps) timed with HIP
+                events on its own stream: algorithmic bytes / mean launch time
+  forms         Gpts/s of all three rotation forms
+  cpu_baseline  the FP32 CPU restatement of the reference's OpenMP loop (the oracle,
+                a port) on the host cores, on a bounded number of steps of the same
+                workload, plus a bitwise comparison of the GPU wavefield after the
+                same steps
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tti-fdmodeling_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per updated point per step (state R+W and parameter words, FP32):
+#   velocity sweep: 5 state reads + 2 writes + params; stress sweep: 5 reads + 3 (6 tensor) writes + params
+SWEEP_BYTES = {
+    "bond": {"v": 4 * (7 + 3), "s": 4 * (8 + 8)},
+    "coord": {"v": 4 * (7 + 5), "s": 4 * (8 + 8)},
+    "tensor": {"v": 4 * (7 + 3), "s": 4 * (11 + 9)},
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(nt):
+    from fdmod_hip import synth
+
+    cfg = dict(synth.CONFIGS["large"])
+    cfg["nt"] = nt
+    m = synth.model_large(nz=cfg["nz"], nx=cfg["nx"], dx=cfg["dx"])
+    src = synth.ricker(cfg["fpeak"], cfg["dt"], nt)
+    return cfg, m, src
+
+
+def make(cls, form, cfg, m, src):
+    o = cls(form, cfg["nz"], cfg["nx"], cfg["nt"], cfg["dx"], cfg["dz"], cfg["dt"], ealThick=cfg["ealThick"],
+            eal_alpha=-1.0, freesurface=cfg["freesurface"], fpeak=cfg["fpeak"], scalar=1.0, mode=1)
+    o.set_model(m["vp"], m["vs"], m["rho"], m["eps"], m["delta"], m["theta"])
+    o.set_source(1, [cfg["sx"]], [cfg["sz"]], src[None])
+    o.set_receivers(2, 1, np.arange(cfg["nx"]), np.full(cfg["nx"], cfg["rz"]))
+    return o
+
+
+def gpu_sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def time_form(form, cfg, m, src, steps, warmup):
+    from fdmod_hip import Modeling
+
+    g = make(Modeling, form, cfg, m, src)
+    pts = (g.ixEnd - g.ixStart) * (g.izEnd - g.izStart)
+    if warmup:
+        g.run(0, warmup)
+    g.reset()
+    gpu_sync()
+    t0 = time.perf_counter()
+    dev_s = g.run(0, steps)
+    gpu_sync()
+    wall = time.perf_counter() - t0
+    return g, pts, wall, dev_s
+
+
+def cpu_baseline(cfg, m, src, form, steps, g_fields):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    cores = int(os.environ.get("FDMOD_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    oracle.set_threads(cores)
+    o = make(oracle.Oracle, form, cfg, m, src)
+    sec = o.run(0, steps)
+    pts = (o.ixEnd - o.ixStart) * (o.izEnd - o.izStart)
+    res = {"value": round(pts * steps / sec / 1e9, 4), "unit": "Gpts/s", "cores": cores, "kind": "port",
+           "sample": f"first {steps} time steps of the same {form} shot (loop time only, omp_get_wtime)",
+           "seconds": round(sec, 3)}
+    same = True
+    worst = 0.0
+    for name, ga in g_fields.items():
+        oa = o.get(name).astype(np.float32)
+        eq = (ga == oa) | (np.isnan(ga) & np.isnan(oa))
+        same &= bool(eq.all())
+        d = np.abs(ga.astype(np.float64) - oa)
+        d = d[np.isfinite(d)]
+        if d.size:
+            worst = max(worst, float(d.max()))
+    parity = {"steps": steps, "fields": sorted(g_fields), "bitwise_equal": same, "max_abs_diff": worst}
+    return res, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--form", default="bond", choices=["bond", "coord", "tensor"])
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU shot sharding is a design note only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    nt = max(args.steps, args.cpu_steps, 1)
+    log(f"bench: building the 1126x7872 workload (nt={nt})")
+    cfg, m, src = workload(nt)
+
+    log(f"bench: timing {args.form}, {args.steps} steps after {args.warmup} warm-up steps")
+    g, pts, wall, dev_s = time_form(args.form, cfg, m, src, args.steps, args.warmup)
+    value = pts * args.steps / wall / 1e9
+    p_tr = g.traces(4)
+    finite = bool(np.isfinite(p_tr[:, :args.steps]).all())
+
+    log("bench: per-sweep HIP-event timing")
+    g.reset()
+    v_ms, s_ms = g.time_sweeps(30)
+    sb = SWEEP_BYTES[args.form]
+    achieved = sb["s"] * pts / (s_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": f"k_ssweep<{args.form}>", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_point": sb["s"], "points_per_launch": pts, "ms_per_launch": round(s_ms, 5),
+                "v_sweep": {"ms_per_launch": round(v_ms, 5), "bytes_per_point": sb["v"],
+                            "achieved": round(sb["v"] * pts / (v_ms * 1e-3) / 1e9, 1)},
+                "step_bytes_per_point": sb["v"] + sb["s"],
+                "step_achieved": round((sb["v"] + sb["s"]) * pts * args.steps / wall / 1e9, 1)}
+    tf = os.path.join(ROOT, "profiles", f"traffic_ssweep_{args.form}_1126x7872.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+
+    forms = {}
+    if not args.no_forms:
+        for fm in ("bond", "coord", "tensor"):
+            if fm == args.form:
+                forms[fm] = round(value, 3)
+                continue
+            log(f"bench: timing form {fm} (1000 steps)")
+            n = min(1000, args.steps)
+            gg, pp, ww, _ = time_form(fm, cfg, m, src, n, 50)
+            forms[fm] = round(pp * n / ww / 1e9, 3)
+            gg.close()
+
+    cpu = None
+    parity = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        log(f"bench: GPU wavefield after {args.cpu_steps} steps for the bitwise check")
+        g.reset()
+        g.run(0, args.cpu_steps)
+        fields = {k: g.get(k) for k in ("vx", "vz", "sxx", "szz", "sxz")}
+        log(f"bench: CPU baseline (oracle FP32 port), {args.cpu_steps} steps")
+        cpu, parity = cpu_baseline(cfg, m, src, args.form, args.cpu_steps, fields)
+
+    out = {
+        "metric": "Gpts/s/step batched vs single-shot; gradients/hr at <=1e-3 rel err vs FP64",
+        "value": round(value, 4),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded marine TTI model, Ricker source)",
+        "config": {"workload": f"plain forward single shot, {args.form} form, nz=1126 nx=7872 dx=dz=10m "
+                               f"dt=0.8ms ealThick=60 free surface, 7872 p receivers",
+                   "grid_padded": [g.Nz, g.Nx], "updated_points_per_step": pts, "parallelism": "single GPU"},
+        "device_seconds": round(dev_s, 4),
+        "traces_finite": finite,
+        "roofline": roofline,
+        "forms_gpts": forms,
+        "cpu_baseline": cpu,
+        "parity_full_size": parity,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -134,6 +134,12 @@ int fdmod_get_energy(fdmod_ctx *ctx, float *E, float *Ek, float *Ep);
 int fdmod_get_array(fdmod_ctx *ctx, int which, float *out);
 int fdmod_set_array(fdmod_ctx *ctx, int which, const float *in);
 
+/* Measurement hook (no counterpart in the reference): launches the velocity sweep
+ * reps times, then the stress sweep reps times, on the context's stream, and returns
+ * the mean duration of one launch of each from HIP events recorded on that stream.
+ * The wavefield advances; call fdmod_reset() before reusing the context. */
+int fdmod_time_sweeps(fdmod_ctx *ctx, int reps, double *ms_v, double *ms_s);
+
 #ifdef __cplusplus
 }
 #endif

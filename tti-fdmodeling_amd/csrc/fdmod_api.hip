@@ -604,3 +604,25 @@ extern "C" int fdmod_set_array(fdmod_ctx *c, int which, const float *in)
   }
   return FDMOD_OK;
 }
+
+extern "C" int fdmod_time_sweeps(fdmod_ctx *c, int reps, double *ms_v, double *ms_s)
+{
+  if (!c || reps < 1) return FDMOD_EINVAL;
+  if (!c->model_set) return fail(c, FDMOD_ESTATE, "fdmod_time_sweeps before fdmod_set_model");
+  SweepArgs sa = sweep_args(c);
+  float ms = 0.0f;
+  HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; r++) launch_vsweep(c->stream, c->p.form, sa);
+  HIPCALL(c, hipEventRecord(c->ev1, c->stream));
+  HIPCALL(c, hipEventSynchronize(c->ev1));
+  HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (ms_v) *ms_v = (double)ms / reps;
+  HIPCALL(c, hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; r++) launch_ssweep(c->stream, c->p.form, sa);
+  HIPCALL(c, hipEventRecord(c->ev1, c->stream));
+  HIPCALL(c, hipEventSynchronize(c->ev1));
+  HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (ms_s) *ms_s = (double)ms / reps;
+  HIPCALL(c, hipGetLastError());
+  return FDMOD_OK;
+}

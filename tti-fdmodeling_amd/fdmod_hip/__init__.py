@@ -126,3 +126,9 @@ class Modeling:
         if a.size != self.Nx * self.Nz:
             raise ValueError("padded arrays must have Nx*Nz samples")
         self._check(self.L.fdmod_set_array(self.c, AID[name], _ptr(a, ct.c_float)))
+
+    def time_sweeps(self, reps=20):
+        """Mean device ms of one velocity-sweep and one stress-sweep launch."""
+        v, s = ct.c_double(0), ct.c_double(0)
+        self._check(self.L.fdmod_time_sweeps(self.c, reps, ct.byref(v), ct.byref(s)))
+        return v.value, s.value

@@ -52,6 +52,7 @@ SIGNATURES = {
     "fdmod_get_energy": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp]),
     "fdmod_get_array": (ct.c_int, [ct.c_void_p, ct.c_int, _fp]),
     "fdmod_set_array": (ct.c_int, [ct.c_void_p, ct.c_int, _fp]),
+    "fdmod_time_sweeps": (ct.c_int, [ct.c_void_p, ct.c_int, _dp, _dp]),
 }
 
 _LIB = None

@@ -69,8 +69,8 @@ def model_large(nz=1126, nx=7872, dx=10.0, seed=2, water=300.0):
     zi = np.empty((nif, nx))
     dip = np.empty((nif, nx))
     for k in range(nif):
-        amp = rng.uniform(60.0, 400.0)
-        lam = rng.uniform(3000.0, 20000.0)
+        amp = rng.uniform(150.0, 600.0)
+        lam = rng.uniform(2500.0, 12000.0)
         ph = rng.uniform(0, 2 * np.pi)
         amp2 = rng.uniform(20.0, 150.0)
         lam2 = rng.uniform(1500.0, 5000.0)
