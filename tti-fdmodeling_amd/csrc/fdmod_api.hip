@@ -4,6 +4,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -140,8 +141,18 @@ extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
   g.ixEnd = g.Nx - kN;
   g.izStart = p->freesurface == 0 ? kN : g.e;
   g.izEnd = g.Nz - kN;
+  g.segLV = kDefaultSegLV;
+  g.segLS = kDefaultSegLS;
+  if (const char *sl = getenv("FDMOD_SEGLV")) {  // tile-shape overrides (tuning experiments)
+    int v = atoi(sl);
+    if (v >= 1 && v <= 4096) g.segLV = v;
+  }
+  if (const char *sl = getenv("FDMOD_SEGLS")) {
+    int v = atoi(sl);
+    if (v >= 1 && v <= 4096) g.segLS = v;
+  }
   g.tilesZ = (g.izEnd - g.izStart + TZ - 1) / TZ;
-  g.tilesX = (g.ixEnd - g.ixStart + TX - 1) / TX;
+  g.flagsX = (g.ixEnd - g.ixStart + kFlagW - 1) / kFlagW;
   g.invdx = (float)(1.0 / p->dx);
   g.invdz = (float)(1.0 / p->dz);
   g.dt = p->dt;
@@ -161,7 +172,7 @@ extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
   int k = 0;
   for (int a = 0; a < FDMOD_A_COUNT; a++) c->d.a[a] = form_uses(p->form, a) ? c->pool + per * (k++) : nullptr;
   if (hipMemsetAsync(c->pool, 0, per * nused * sizeof(float), c->stream) != hipSuccess ||
-      hipMalloc(&c->flags, (size_t)g.tilesZ * g.tilesX + 1) != hipSuccess) {
+      hipMalloc(&c->flags, (size_t)g.tilesZ * g.flagsX + 1) != hipSuccess) {
     fdmod_destroy(c);
     return FDMOD_ENOMEM;
   }

@@ -21,20 +21,23 @@ constexpr float kC5 = 0.00132022981661381f;
 // PI as defined by Seismic Unix cwp.h, used by the theta conversion (BT:508).
 constexpr double kPI = 3.141592653589793;
 
-// Sweep tile: TZ rows (one per lane of a wave) by TX columns; 4 waves per block,
-// each wave owns TX/4 columns.  Both sweeps use the same tiling so that one
-// per-tile "undamped" flag serves both.
+// Sweep decomposition: one wavefront per block owns a strip of TZ = 64 rows (one
+// row per lane) and marches along x over a segment of columns (segLV columns in
+// the velocity sweep, segLS in the stress sweep).  "Undamped" flags are kept per
+// fine tile of TZ x kFlagW points; a block ANDs the flags its segment covers.
 constexpr int TZ = 64;
-constexpr int TX = 32;
-constexpr int kThreads = 256;
-constexpr int kColsPerWave = TX / (kThreads / 64);
+constexpr int kRing = 11;          // LDS ring of staged columns (10 live + 1 in flight)
+constexpr int kFlagW = 11;         // columns per fine flag tile
+constexpr int kDefaultSegLV = 44;  // measured best on a 1256 x 8002 grid
+constexpr int kDefaultSegLS = 11;
 
 // Device-side geometry of a padded grid.
 struct Grid {
   int Nz, Nx, ldz;         // padded sizes; ldz = column pitch in floats (>= Nz)
   int ixStart, ixEnd, izStart, izEnd;
   int e;                   // effective ealThick (user value + kN)
-  int tilesZ, tilesX;      // sweep tiles over the update region
+  int segLV, segLS;        // columns per x segment of the velocity / stress sweep
+  int tilesZ, flagsX;      // 64-row strips; fine flag tiles along x
   float invdx, invdz, dt;  // float(1.0/dx), float(1.0/dz), dt
 };
 

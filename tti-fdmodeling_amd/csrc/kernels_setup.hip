@@ -228,9 +228,9 @@ __global__ void k_tile_flags(Grid g, const float *__restrict__ coef1, const floa
                              uint8_t *flags)
 {
   int tz = blockIdx.x, tx = blockIdx.y;
-  int z0 = g.izStart + tz * TZ, x0 = g.ixStart + tx * TX;
+  int z0 = g.izStart + tz * TZ, x0 = g.ixStart + tx * kFlagW;
   int ok = 1;
-  for (int idx = threadIdx.x; idx < TZ * TX; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < TZ * kFlagW; idx += blockDim.x) {
     int ix = x0 + idx / TZ, iz = z0 + idx % TZ;
     if (ix < g.ixEnd && iz < g.izEnd) {
       size_t o = (size_t)ix * g.ldz + iz;
@@ -243,7 +243,7 @@ __global__ void k_tile_flags(Grid g, const float *__restrict__ coef1, const floa
 
 void launch_tile_flags(hipStream_t s, const Grid &g, const float *coef1, const float *coef2, uint8_t *flags)
 {
-  hipLaunchKernelGGL(k_tile_flags, dim3(g.tilesZ, g.tilesX), dim3(256), 0, s, g, coef1, coef2, flags);
+  hipLaunchKernelGGL(k_tile_flags, dim3(g.tilesZ, g.flagsX), dim3(256), 0, s, g, coef1, coef2, flags);
 }
 
 }  // namespace fdm

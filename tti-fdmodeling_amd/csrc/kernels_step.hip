@@ -5,214 +5,409 @@
 // Arithmetic follows the reference's float expressions term by term with FP
 // contraction disabled at build time, so each updated value is the one the
 // reference's C computes.
+//
+// Sweep structure.  One wavefront per block owns a strip of 64 rows (lane = row)
+// and marches along x over a segment of columns.  The staged field(s) of the
+// stencil live in an LDS ring of kRing columns of 73 rows (64 + the 9-row halo).
+// Each iteration stores the column loaded one iteration earlier into the ring,
+// issues the loads of the next column and of the next column's per-point
+// operands, then evaluates one output column from the ring.  Column loads use a
+// wave-uniform base (scalar address) plus the lane's row, so staging costs almost
+// no vector ALU work, and memory traffic overlaps the stencil arithmetic of the
+// column in flight.  The ring index of every stencil tap is a compile-time
+// constant inside a kRing-times unrolled iteration block.
+#include <type_traits>
+#include <utility>
+
 #include "fdmod_internal.h"
 
 namespace fdm {
 
-constexpr int LZ = TZ + 2 * kN - 1;  // staged rows per tile column (halo 4/5)
-constexpr int LX = TX + 2 * kN - 1;  // staged columns per tile
+constexpr int LZ = TZ + 2 * kN - 1;  // staged rows per column: 64 + 9 halo
+constexpr int kHalo = LZ - TZ;       // rows loaded by the first kHalo lanes a second time
 
-// Stage one field's halo tile into LDS: columns gx0 .. gx0+LX-1, rows gz0 .. gz0+LZ-1.
-// Points outside the padded grid are never read by a valid output; they are set to 0.
-__device__ __forceinline__ void stage(float (*sh)[LZ], const float *__restrict__ f, int gx0, int gz0, const Grid &g)
+// Cache policy (measured on MI355X, 1256 x 8002 grid): default caching for the
+// staged stencil columns, the velocity sweep's operands and all outputs; the stress
+// sweep's read-once operands (old stress, stiffness, coefs) load non-temporally.
+// Making every stream non-temporal was slowest (V 105 us, S 170 us); this mix gave
+// V 65 us and S 116 us.
+__device__ __forceinline__ float ldnt(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ldst(const float *p) { return *p; }
+__device__ __forceinline__ void stnt(float *p, float v) { *p = v; }
+
+// AND of the fine "undamped" flags covering columns [xa, xa+nc) of strip tz.
+__device__ __forceinline__ bool segment_undamped(const Grid &g, const uint8_t *flags, int tz, int xa, int nc)
 {
-  for (int idx = threadIdx.x; idx < LX * LZ; idx += kThreads) {
-    int cx = idx / LZ;
-    int rz = idx - cx * LZ;
-    int gx = gx0 + cx, gz = gz0 + rz;
-    float v = 0.0f;
-    if (gx < g.Nx && gz < g.Nz) v = f[(size_t)gx * g.ldz + gz];
-    sh[cx][rz] = v;
+  bool und = true;
+  const int f0 = (xa - g.ixStart) / kFlagW, f1 = (xa + nc - 1 - g.ixStart) / kFlagW;
+  for (int f = f0; f <= f1; f++) und = und && flags[f * g.tilesZ + tz] != 0;
+  return und;
+}
+
+template <int U, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+  if constexpr (U < N) {
+    f(std::integral_constant<int, U>{});
+    static_for<U + 1, N>(f);
   }
 }
 
-// A1: stress -> velocity point derivative pair at LDS position (X, Z) of (ix, iz),
-// d1 along the (+x,+z) diagonal and d2 along the (+x,-z) diagonal (BT:661-699).
-__device__ __forceinline__ void rsg_fwd(const float (*S)[LZ], int X, int Z, float &d1, float &d2)
+// Map a 1-D block id to (strip, segment) so that each XCD works on a contiguous
+// run of tiles (strips fastest, then segments).  Blocks are dealt to the 8 XCDs
+// round-robin, so block b runs on XCD b % 8; giving XCD k the k-th contiguous
+// chunk keeps neighbouring tiles -- and the halo they share -- in one L2.
+constexpr int kXCD = 8;
+__device__ __forceinline__ int tiles_x(const Grid &g, int segL) { return (g.ixEnd - g.ixStart + segL - 1) / segL; }
+
+__device__ __forceinline__ bool tile_of_block(const Grid &g, int segL, int &tz, int &tx)
 {
-  d1 = kC1 * (S[X + 1][Z + 1] - S[X][Z]) + kC2 * (S[X + 2][Z + 2] - S[X - 1][Z - 1]) +
-       kC3 * (S[X + 3][Z + 3] - S[X - 2][Z - 2]) + kC4 * (S[X + 4][Z + 4] - S[X - 3][Z - 3]) +
-       kC5 * (S[X + 5][Z + 5] - S[X - 4][Z - 4]);
-  d2 = kC1 * (S[X + 1][Z] - S[X][Z + 1]) + kC2 * (S[X + 2][Z - 1] - S[X - 1][Z + 2]) +
-       kC3 * (S[X + 3][Z - 2] - S[X - 2][Z + 3]) + kC4 * (S[X + 4][Z - 3] - S[X - 3][Z + 4]) +
-       kC5 * (S[X + 5][Z - 4] - S[X - 4][Z + 5]);
+  const int n = g.tilesZ * tiles_x(g, segL);
+  const int chunk = (n + kXCD - 1) / kXCD;
+  const int b = blockIdx.x;
+  const int t = (b % kXCD) * chunk + b / kXCD;
+  if (t >= n) return false;
+  tx = t / g.tilesZ;
+  tz = t - tx * g.tilesZ;
+  return true;
 }
 
-// A2: velocity -> stress derivative pair (BT:843-861).
-__device__ __forceinline__ void rsg_bwd(const float (*V)[LZ], int X, int Z, float &d1, float &d2)
+// One staged column of NF fields: lane l holds row rs+l and, for l < kHalo, row rs+64+l.
+template <int NF>
+struct Col {
+  float lo[NF], hi[NF];
+};
+
+template <int NF>
+__device__ __forceinline__ void load_col(Col<NF> &c, const float *const (&f)[NF], int gx, int rs, int lane,
+                                         const Grid &g)
 {
-  d1 = kC1 * (V[X][Z] - V[X - 1][Z - 1]) + kC2 * (V[X + 1][Z + 1] - V[X - 2][Z - 2]) +
-       kC3 * (V[X + 2][Z + 2] - V[X - 3][Z - 3]) + kC4 * (V[X + 3][Z + 3] - V[X - 4][Z - 4]) +
-       kC5 * (V[X + 4][Z + 4] - V[X - 5][Z - 5]);
-  d2 = kC1 * (V[X][Z - 1] - V[X - 1][Z]) + kC2 * (V[X + 1][Z - 2] - V[X - 2][Z + 1]) +
-       kC3 * (V[X + 2][Z - 3] - V[X - 3][Z + 2]) + kC4 * (V[X + 3][Z - 4] - V[X - 4][Z + 3]) +
-       kC5 * (V[X + 4][Z - 5] - V[X - 5][Z + 4]);
+  const size_t base = (size_t)gx * g.ldz;
+  const int r0 = rs + lane, r1 = rs + TZ + lane;
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    const float *col = f[k] + base;
+    c.lo[k] = r0 < g.Nz ? ldst(col + r0) : 0.0f;
+    c.hi[k] = (lane < kHalo && r1 < g.Nz) ? ldst(col + r1) : 0.0f;
+  }
+}
+
+template <int NF>
+__device__ __forceinline__ void store_col(float (*ring)[kRing][LZ], int slot, const Col<NF> &c, int lane)
+{
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    ring[k][slot][lane] = c.lo[k];
+    if (lane < kHalo) ring[k][slot][TZ + lane] = c.hi[k];
+  }
+}
+
+// A1 (BT:661-699): stress -> velocity pair at the output point, reading column
+// offset dx and row offset dz through T(dx, dz).  d1 runs along (+x,+z), d2 along (+x,-z).
+template <typename T>
+__device__ __forceinline__ void rsg_fwd(T S, float &d1, float &d2)
+{
+  d1 = kC1 * (S(1, 1) - S(0, 0)) + kC2 * (S(2, 2) - S(-1, -1)) + kC3 * (S(3, 3) - S(-2, -2)) +
+       kC4 * (S(4, 4) - S(-3, -3)) + kC5 * (S(5, 5) - S(-4, -4));
+  d2 = kC1 * (S(1, 0) - S(0, 1)) + kC2 * (S(2, -1) - S(-1, 2)) + kC3 * (S(3, -2) - S(-2, 3)) +
+       kC4 * (S(4, -3) - S(-3, 4)) + kC5 * (S(5, -4) - S(-4, 5));
+}
+
+// A2 (BT:843-861): velocity -> stress pair.
+template <typename T>
+__device__ __forceinline__ void rsg_bwd(T V, float &d1, float &d2)
+{
+  d1 = kC1 * (V(0, 0) - V(-1, -1)) + kC2 * (V(1, 1) - V(-2, -2)) + kC3 * (V(2, 2) - V(-3, -3)) +
+       kC4 * (V(3, 3) - V(-4, -4)) + kC5 * (V(4, 4) - V(-5, -5));
+  d2 = kC1 * (V(0, -1) - V(-1, 0)) + kC2 * (V(1, -2) - V(-2, 1)) + kC3 * (V(2, -3) - V(-3, 2)) +
+       kC4 * (V(3, -4) - V(-4, 3)) + kC5 * (V(4, -5) - V(-5, 4));
+}
+
+// ---------------------------------------------------------------- velocity sweep
+struct VOps {
+  float vx, vz, ir, k1, k2, cv, sv;
+};
+
+template <bool COORD>
+__device__ __forceinline__ void load_vops(VOps &p, const SweepArgs &a, size_t o, bool und, float dt)
+{
+  p.vx = a.vx[o];
+  p.vz = a.vz[o];
+  p.ir = a.invrho[o];
+  p.k1 = und ? 1.0f : a.coef1[o];
+  p.k2 = und ? dt : a.coef2[o];
+  if (COORD) {
+    p.cv = a.csv[o];
+    p.sv = a.snv[o];
+  } else {
+    p.cv = p.sv = 0.0f;
+  }
 }
 
 // Velocity sweep (A1 + A3).  Bond and tensor: BT:654-704 / TR:617-666.
 // Coordinate (COORD): CR:614-671, with the reference's invdx on d(sxx)/dz.
-template <bool COORD>
-__global__ __launch_bounds__(kThreads) void k_vsweep(SweepArgs a)
+// The loads of the staged column and of the per-point operands run PD iterations
+// ahead of their use.
+template <bool COORD, int PD>
+__global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
 {
-  __shared__ float sh[3][LX][LZ];
+  __shared__ float ring[3][kRing][LZ];
   const Grid &g = a.g;
-  const int tz = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int z0 = g.izStart + blockIdx.x * TZ;
-  const int x0 = g.ixStart + blockIdx.y * TX;
-  stage(sh[0], a.sxx, x0 - 4, z0 - 4, g);
-  stage(sh[1], a.szz, x0 - 4, z0 - 4, g);
-  stage(sh[2], a.sxz, x0 - 4, z0 - 4, g);
-  __syncthreads();
-  const bool und = a.undamped[blockIdx.y * g.tilesZ + blockIdx.x] != 0;
-  const int iz = z0 + tz;
+  const int segL = g.segLV;
+  int btz, btx;
+  if (!tile_of_block(g, segL, btz, btx)) return;
+  const int lane = threadIdx.x;
+  const int z0 = g.izStart + btz * TZ;
+  const int xa = g.ixStart + btx * segL;
+  const int nc = min(segL, g.ixEnd - xa);
+  const bool und = segment_undamped(g, a.undamped, btz, xa, nc);
+  const int iz = z0 + lane;
+  const bool zok = iz < g.izEnd;
+  const int rs = z0 - 4;  // first staged row
+  const float *const fs[3] = {a.sxx, a.szz, a.sxz};
+  // prologue: columns xa-4 .. xa+4 into slots 0..8; the next PD columns and operands in flight
+  {
+    Col<3> c[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) load_col<3>(c[m], fs, xa - 4 + m, rs, lane, g);
+#pragma unroll
+    for (int m = 0; m < 9; m++) store_col<3>(ring, m, c[m], lane);
+  }
+  Col<3> colq[PD];
+  VOps opq[PD];
+#pragma unroll
+  for (int i = 0; i < PD; i++) {
+    if (i <= nc - 1) load_col<3>(colq[i], fs, xa + 5 + i, rs, lane, g);
+    if (zok && i < nc) load_vops<COORD>(opq[i], a, (size_t)(xa + i) * g.ldz + iz, und, g.dt);
+  }
   const float invdx = g.invdx, invdz = g.invdz;
-#pragma unroll 1
-  for (int j = 0; j < kColsPerWave; j++) {
-    const int cx = w * kColsPerWave + j;
-    const int ix = x0 + cx;
-    if (ix >= g.ixEnd || iz >= g.izEnd) continue;
-    const int X = cx + 4, Z = tz + 4;
-    const size_t o = (size_t)ix * g.ldz + iz;
-    float d1, d2;
-    rsg_fwd(sh[0], X, Z, d1, d2);
-    float sxxdx = 0.5f * (d1 + d2) * invdx;
-    float sxxdz = 0.0f;
-    if (COORD) sxxdz = 0.5f * (d1 - d2) * invdx;
-    rsg_fwd(sh[1], X, Z, d1, d2);
-    float szzdx = 0.0f;
-    if (COORD) szzdx = 0.5f * (d1 + d2) * invdx;
-    float szzdz = 0.5f * (d1 - d2) * invdz;
-    rsg_fwd(sh[2], X, Z, d1, d2);
-    float sxzdx = 0.5f * (d1 + d2) * invdx;
-    float sxzdz = 0.5f * (d1 - d2) * invdz;
-    const float k1 = und ? 1.0f : a.coef1[o];
-    const float k2 = und ? g.dt : a.coef2[o];
-    const float ir = a.invrho[o];
-    const float kr = k2 * ir;
-    if (!COORD) {
-      a.vx[o] = k1 * a.vx[o] + kr * (sxxdx + sxzdz);
-      a.vz[o] = k1 * a.vz[o] + kr * (sxzdx + szzdz);
-    } else {
-      const float cv = a.csv[o], sv = a.snv[o];
-      a.vx[o] = k1 * a.vx[o] + kr * (cv * sxxdx - sv * sxxdz + sv * sxzdx + cv * sxzdz);
-      a.vz[o] = k1 * a.vz[o] + kr * (cv * sxzdx - sv * sxzdz + sv * szzdx + cv * szzdz);
-    }
+  for (int k0 = 0; k0 < nc; k0 += kRing) {
+    static_for<0, kRing>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int k = k0 + u;
+      if (k >= nc) return;
+      const int c = xa + k;
+      Col<3> cn;
+      VOps on;
+      if (k <= nc - PD - 1) load_col<3>(cn, fs, c + 5 + PD, rs, lane, g);
+      if (zok && k + PD < nc) load_vops<COORD>(on, a, (size_t)(c + PD) * g.ldz + iz, und, g.dt);
+      store_col<3>(ring, (u + 9) % kRing, colq[0], lane);
+      __syncthreads();
+      if (zok) {
+        const VOps &op = opq[0];
+        float d1, d2;
+        rsg_fwd([&](int dx, int dz) { return ring[0][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        const float sxxdx = 0.5f * (d1 + d2) * invdx;
+        const float sxxdz = COORD ? 0.5f * (d1 - d2) * invdx : 0.0f;
+        rsg_fwd([&](int dx, int dz) { return ring[1][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        const float szzdx = COORD ? 0.5f * (d1 + d2) * invdx : 0.0f;
+        const float szzdz = 0.5f * (d1 - d2) * invdz;
+        rsg_fwd([&](int dx, int dz) { return ring[2][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        const float sxzdx = 0.5f * (d1 + d2) * invdx;
+        const float sxzdz = 0.5f * (d1 - d2) * invdz;
+        const size_t o = (size_t)c * g.ldz + iz;
+        const float kr = op.k2 * op.ir;
+        if (!COORD) {
+          stnt(a.vx + o, op.k1 * op.vx + kr * (sxxdx + sxzdz));
+          stnt(a.vz + o, op.k1 * op.vz + kr * (sxzdx + szzdz));
+        } else {
+          const float cv = op.cv, sv = op.sv;
+          stnt(a.vx + o, op.k1 * op.vx + kr * (cv * sxxdx - sv * sxxdz + sv * sxzdx + cv * sxzdz));
+          stnt(a.vz + o, op.k1 * op.vz + kr * (cv * sxzdx - sv * sxzdz + sv * szzdx + cv * szzdz));
+        }
+      }
+#pragma unroll
+      for (int i = 0; i + 1 < PD; i++) {
+        colq[i] = colq[i + 1];
+        opq[i] = opq[i + 1];
+      }
+      colq[PD - 1] = cn;
+      opq[PD - 1] = on;
+    });
   }
 }
 
+// ---------------------------------------------------------------- stress sweep
+// Per-form operands of one stress-sweep point: old state, stiffness, trig, coefs.
+struct SOps {
+  float s1, s2, s3;  // old sxx, szz, sxz (Bond/coord) or txx, tzz, txz (tensor)
+  float c11, c13, c33, c55;
+  float a, b, c;     // Bond: c15, c35, -; coord: sin, cos, -; tensor: sin2, cos2, sin2t
+  float k1, k2;
+};
+
+template <int FORM>
+__device__ __forceinline__ void load_sops(SOps &p, const SweepArgs &a, size_t o, bool und, float dt)
+{
+  if (FORM == FDMOD_FORM_TENSOR) {
+    p.s1 = ldnt(a.txx + o); p.s2 = ldnt(a.tzz + o); p.s3 = ldnt(a.txz + o);
+  } else {
+    p.s1 = ldnt(a.sxx + o); p.s2 = ldnt(a.szz + o); p.s3 = ldnt(a.sxz + o);
+  }
+  p.c11 = ldnt(a.c11 + o); p.c13 = ldnt(a.c13 + o); p.c33 = ldnt(a.c33 + o); p.c55 = ldnt(a.c55 + o);
+  if (FORM == FDMOD_FORM_BOND) { p.a = ldnt(a.c15 + o); p.b = ldnt(a.c35 + o); p.c = 0.0f; }
+  else if (FORM == FDMOD_FORM_COORD) { p.a = ldnt(a.sn + o); p.b = ldnt(a.cs + o); p.c = 0.0f; }
+  else { p.a = ldnt(a.sin2 + o); p.b = ldnt(a.cos2 + o); p.c = ldnt(a.sin2t + o); }
+  p.k1 = und ? 1.0f : ldnt(a.coef1 + o);
+  p.k2 = und ? dt : ldnt(a.coef2 + o);
+}
+
 // Stress sweep (A2 + A4/A5/A6), with the free-surface stress imaging (BT:875-891)
-// done by the top row of tiles, whose first rows are izStart .. izStart+5.
+// done by the top strip, whose first rows are izStart .. izStart+5.
 //   FORM_BOND   BT:836-873
 //   FORM_TENSOR TR:799-836 and the sigma <- R(tau) sweep TR:839-852, fused pointwise
 //   FORM_COORD  CR:804-845
-template <int FORM>
-__global__ __launch_bounds__(kThreads) void k_ssweep(SweepArgs a)
+template <int FORM, int PD>
+__global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
 {
-  __shared__ float sh[2][LX][LZ];
+  __shared__ float ring[2][kRing][LZ];
   const Grid &g = a.g;
-  const int tz = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int z0 = g.izStart + blockIdx.x * TZ;
-  const int x0 = g.ixStart + blockIdx.y * TX;
-  stage(sh[0], a.vx, x0 - 5, z0 - 5, g);
-  stage(sh[1], a.vz, x0 - 5, z0 - 5, g);
-  __syncthreads();
-  const bool und = a.undamped[blockIdx.y * g.tilesZ + blockIdx.x] != 0;
-  const bool fs_top = a.freesurface && blockIdx.x == 0;
-  const int iz = z0 + tz;
-  const float invdx = g.invdx, invdz = g.invdz;
-#pragma unroll 1
-  for (int j = 0; j < kColsPerWave; j++) {
-    const int cx = w * kColsPerWave + j;
-    const int ix = x0 + cx;
-    if (ix >= g.ixEnd) break;  // wave-uniform
-    const bool valid = iz < g.izEnd;
-    const size_t o = (size_t)ix * g.ldz + iz;
-    float nxx = 0.0f, nzz = 0.0f, nxz = 0.0f;
-    if (valid) {
-      const int X = cx + 5, Z = tz + 5;
-      float d1, d2;
-      rsg_bwd(sh[0], X, Z, d1, d2);
-      float vxdx = 0.5f * (d1 + d2) * invdx;
-      float vxdz = 0.5f * (d1 - d2) * invdz;
-      rsg_bwd(sh[1], X, Z, d1, d2);
-      float vzdx = 0.5f * (d1 + d2) * invdx;
-      float vzdz = 0.5f * (d1 - d2) * invdz;
-      const float k1 = und ? 1.0f : a.coef1[o];
-      const float k2 = und ? g.dt : a.coef2[o];
-      if (FORM == FDMOD_FORM_BOND) {
-        const float c11 = a.c11[o], c13 = a.c13[o], c15 = a.c15[o];
-        const float c33 = a.c33[o], c35 = a.c35[o], c55 = a.c55[o];
-        nxx = k1 * a.sxx[o] + k2 * (c11 * vxdx + c13 * vzdz + c15 * (vzdx + vxdz));
-        nzz = k1 * a.szz[o] + k2 * (c13 * vxdx + c33 * vzdz + c35 * (vzdx + vxdz));
-        nxz = k1 * a.sxz[o] + k2 * (c15 * vxdx + c35 * vzdz + c55 * (vzdx + vxdz));
-      } else if (FORM == FDMOD_FORM_COORD) {
-        const float sn = a.sn[o], cs = a.cs[o];
-        float t = sn * vxdx + cs * vxdz;
-        vxdx = cs * vxdx - sn * vxdz;
-        vxdz = t;
-        t = sn * vzdx + cs * vzdz;
-        vzdx = cs * vzdx - sn * vzdz;
-        vzdz = t;
-        const float c11 = a.c11[o], c13 = a.c13[o], c33 = a.c33[o], c55 = a.c55[o];
-        nxx = k1 * a.sxx[o] + k2 * (c11 * vxdx + c13 * vzdz);
-        nzz = k1 * a.szz[o] + k2 * (c13 * vxdx + c33 * vzdz);
-        nxz = k1 * a.sxz[o] + k2 * c55 * (vzdx + vxdz);
-      } else {
-        const float S2 = a.sin2[o], K2 = a.cos2[o], S2T = a.sin2t[o];
-        const float exx = K2 * vxdx + S2 * vzdz - 0.5f * S2T * (vxdz + vzdx);
-        const float ezz = S2 * vxdx + K2 * vzdz + 0.5f * S2T * (vxdz + vzdx);
-        const float exz = S2T * (vxdx - vzdz) + (K2 - S2) * (vxdz + vzdx);
-        const float c11 = a.c11[o], c13 = a.c13[o], c33 = a.c33[o], c55 = a.c55[o];
-        const float t1 = k1 * a.txx[o] + k2 * (c11 * exx + c13 * ezz);
-        const float t2 = k1 * a.tzz[o] + k2 * (c13 * exx + c33 * ezz);
-        const float t3 = k1 * a.txz[o] + k2 * c55 * exz;
-        a.txx[o] = t1;
-        a.tzz[o] = t2;
-        a.txz[o] = t3;
-        nxx = K2 * t1 + S2 * t2 + S2T * t3;
-        nzz = S2 * t1 + K2 * t2 - S2T * t3;
-        nxz = -0.5f * S2T * (t1 - t2) + (K2 - S2) * t3;
-      }
-    } else if (fs_top && iz < g.Nz) {
-      // rows past izEnd are not updated; their current values feed the images
-      nxx = a.sxx[o];
-      nzz = a.szz[o];
-      nxz = a.sxz[o];
-    }
-    if (fs_top) {
-      if (tz == 0) {
-        nxz = 0.0f;
-        nzz = 0.0f;
-      }
-      const float s0 = __shfl(nxx, 0, 64);
-      if (tz >= 1 && tz <= kN) {
-        const size_t m = (size_t)ix * g.ldz + (z0 - tz);
-        a.sxx[m] = 2.0f * s0 - nxx;
-        a.szz[m] = -nzz;
-        a.sxz[m] = -nxz;
-      }
-    }
-    if (valid) {
-      a.sxx[o] = nxx;
-      a.szz[o] = nzz;
-      a.sxz[o] = nxz;
-    }
+  const int segL = g.segLS;
+  int btz, btx;
+  if (!tile_of_block(g, segL, btz, btx)) return;
+  const int lane = threadIdx.x;
+  const int z0 = g.izStart + btz * TZ;
+  const int xa = g.ixStart + btx * segL;
+  const int nc = min(segL, g.ixEnd - xa);
+  const bool und = segment_undamped(g, a.undamped, btz, xa, nc);
+  const bool fs_top = a.freesurface && btz == 0;
+  const int iz = z0 + lane;
+  const bool zok = iz < g.izEnd;
+  const int rs = z0 - 5;
+  const float *const fv[2] = {a.vx, a.vz};
+  {
+    Col<2> c[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) load_col<2>(c[m], fv, xa - 5 + m, rs, lane, g);
+#pragma unroll
+    for (int m = 0; m < 9; m++) store_col<2>(ring, m, c[m], lane);
   }
+  Col<2> colq[PD];
+  SOps opq[PD];
+#pragma unroll
+  for (int i = 0; i < PD; i++) {
+    if (i <= nc - 1) load_col<2>(colq[i], fv, xa + 4 + i, rs, lane, g);
+    if (zok && i < nc) load_sops<FORM>(opq[i], a, (size_t)(xa + i) * g.ldz + iz, und, g.dt);
+  }
+  const float invdx = g.invdx, invdz = g.invdz;
+  for (int k0 = 0; k0 < nc; k0 += kRing) {
+    static_for<0, kRing>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int k = k0 + u;
+      if (k >= nc) return;
+      const int c = xa + k;
+      Col<2> cn;
+      SOps on;
+      if (k <= nc - PD - 1) load_col<2>(cn, fv, c + 4 + PD, rs, lane, g);
+      if (zok && k + PD < nc) load_sops<FORM>(on, a, (size_t)(c + PD) * g.ldz + iz, und, g.dt);
+      store_col<2>(ring, (u + 9) % kRing, colq[0], lane);
+      __syncthreads();
+      const size_t o = (size_t)c * g.ldz + iz;
+      float nxx = 0.0f, nzz = 0.0f, nxz = 0.0f;
+      if (zok) {
+        const SOps &op = opq[0];
+        float d1, d2;
+        rsg_bwd([&](int dx, int dz) { return ring[0][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, d1, d2);
+        float vxdx = 0.5f * (d1 + d2) * invdx;
+        float vxdz = 0.5f * (d1 - d2) * invdz;
+        rsg_bwd([&](int dx, int dz) { return ring[1][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, d1, d2);
+        float vzdx = 0.5f * (d1 + d2) * invdx;
+        float vzdz = 0.5f * (d1 - d2) * invdz;
+        const float k1 = op.k1, k2 = op.k2;
+        if (FORM == FDMOD_FORM_BOND) {
+          const float c15 = op.a, c35 = op.b;
+          nxx = k1 * op.s1 + k2 * (op.c11 * vxdx + op.c13 * vzdz + c15 * (vzdx + vxdz));
+          nzz = k1 * op.s2 + k2 * (op.c13 * vxdx + op.c33 * vzdz + c35 * (vzdx + vxdz));
+          nxz = k1 * op.s3 + k2 * (c15 * vxdx + c35 * vzdz + op.c55 * (vzdx + vxdz));
+        } else if (FORM == FDMOD_FORM_COORD) {
+          const float sn = op.a, cs = op.b;
+          float t = sn * vxdx + cs * vxdz;
+          vxdx = cs * vxdx - sn * vxdz;
+          vxdz = t;
+          t = sn * vzdx + cs * vzdz;
+          vzdx = cs * vzdx - sn * vzdz;
+          vzdz = t;
+          nxx = k1 * op.s1 + k2 * (op.c11 * vxdx + op.c13 * vzdz);
+          nzz = k1 * op.s2 + k2 * (op.c13 * vxdx + op.c33 * vzdz);
+          nxz = k1 * op.s3 + k2 * op.c55 * (vzdx + vxdz);
+        } else {
+          const float S2 = op.a, K2 = op.b, S2T = op.c;
+          const float exx = K2 * vxdx + S2 * vzdz - 0.5f * S2T * (vxdz + vzdx);
+          const float ezz = S2 * vxdx + K2 * vzdz + 0.5f * S2T * (vxdz + vzdx);
+          const float exz = S2T * (vxdx - vzdz) + (K2 - S2) * (vxdz + vzdx);
+          const float t1 = k1 * op.s1 + k2 * (op.c11 * exx + op.c13 * ezz);
+          const float t2 = k1 * op.s2 + k2 * (op.c13 * exx + op.c33 * ezz);
+          const float t3 = k1 * op.s3 + k2 * op.c55 * exz;
+          stnt(a.txx + o, t1);
+          stnt(a.tzz + o, t2);
+          stnt(a.txz + o, t3);
+          nxx = K2 * t1 + S2 * t2 + S2T * t3;
+          nzz = S2 * t1 + K2 * t2 - S2T * t3;
+          nxz = -0.5f * S2T * (t1 - t2) + (K2 - S2) * t3;
+        }
+      } else if (fs_top && iz < g.Nz) {
+        // rows past izEnd are not updated; their current values feed the images
+        nxx = a.sxx[o];
+        nzz = a.szz[o];
+        nxz = a.sxz[o];
+      }
+      if (fs_top) {
+        if (lane == 0) {
+          nxz = 0.0f;
+          nzz = 0.0f;
+        }
+        const float s0 = __shfl(nxx, 0, 64);
+        if (lane >= 1 && lane <= kN) {
+          const size_t m = (size_t)c * g.ldz + (z0 - lane);
+          a.sxx[m] = 2.0f * s0 - nxx;
+          a.szz[m] = -nzz;
+          a.sxz[m] = -nxz;
+        }
+      }
+      if (zok) {
+        stnt(a.sxx + o, nxx);
+        stnt(a.szz + o, nzz);
+        stnt(a.sxz + o, nxz);
+      }
+#pragma unroll
+      for (int i = 0; i + 1 < PD; i++) {
+        colq[i] = colq[i + 1];
+        opq[i] = opq[i + 1];
+      }
+      colq[PD - 1] = cn;
+      opq[PD - 1] = on;
+    });
+  }
+}
+
+#ifndef FDMOD_PIPE_V
+#define FDMOD_PIPE_V 2
+#endif
+#ifndef FDMOD_PIPE_S
+#define FDMOD_PIPE_S 2
+#endif
+constexpr int kPipeV = FDMOD_PIPE_V;  // prefetch depth (iterations) of the velocity sweep
+constexpr int kPipeS = FDMOD_PIPE_S;  // and of the stress sweep
+
+static dim3 sweep_grid(const Grid &g, int segL)
+{
+  const int n = g.tilesZ * ((g.ixEnd - g.ixStart + segL - 1) / segL);
+  return dim3((n + kXCD - 1) / kXCD * kXCD);
 }
 
 void launch_vsweep(hipStream_t s, int form, const SweepArgs &a)
 {
-  dim3 grid(a.g.tilesZ, a.g.tilesX);
-  if (form == FDMOD_FORM_COORD) hipLaunchKernelGGL(k_vsweep<true>, grid, dim3(kThreads), 0, s, a);
-  else hipLaunchKernelGGL(k_vsweep<false>, grid, dim3(kThreads), 0, s, a);
+  dim3 grid = sweep_grid(a.g, a.g.segLV);
+  if (form == FDMOD_FORM_COORD) hipLaunchKernelGGL((k_vsweep<true, kPipeV>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((k_vsweep<false, kPipeV>), grid, dim3(64), 0, s, a);
 }
 
 void launch_ssweep(hipStream_t s, int form, const SweepArgs &a)
 {
-  dim3 grid(a.g.tilesZ, a.g.tilesX);
-  if (form == FDMOD_FORM_COORD) hipLaunchKernelGGL(k_ssweep<FDMOD_FORM_COORD>, grid, dim3(kThreads), 0, s, a);
-  else if (form == FDMOD_FORM_TENSOR) hipLaunchKernelGGL(k_ssweep<FDMOD_FORM_TENSOR>, grid, dim3(kThreads), 0, s, a);
-  else hipLaunchKernelGGL(k_ssweep<FDMOD_FORM_BOND>, grid, dim3(kThreads), 0, s, a);
+  dim3 grid = sweep_grid(a.g, a.g.segLS);
+  if (form == FDMOD_FORM_COORD) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_COORD, kPipeS>), grid, dim3(64), 0, s, a);
+  else if (form == FDMOD_FORM_TENSOR) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_TENSOR, kPipeS>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_BOND, kPipeS>), grid, dim3(64), 0, s, a);
 }
 
 // Source injection (BT:706-834 on velocity, BT:893-914 on stress).  Each pass is a
