@@ -33,7 +33,7 @@ constexpr int kHalo = LZ - TZ;       // rows loaded by the first kHalo lanes a s
 // V 65 us and S 116 us.
 __device__ __forceinline__ float ldnt(const float *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ float ldst(const float *p) { return *p; }
-__device__ __forceinline__ void stnt(float *p, float v) { *p = v; }
+__device__ __forceinline__ void st_out(float *p, float v) { *p = v; }
 
 // AND of the fine "undamped" flags covering columns [xa, xa+nc) of strip tz.
 __device__ __forceinline__ bool segment_undamped(const Grid &g, const uint8_t *flags, int tz, int xa, int nc)
@@ -208,12 +208,12 @@ __global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
         const size_t o = (size_t)c * g.ldz + iz;
         const float kr = op.k2 * op.ir;
         if (!COORD) {
-          stnt(a.vx + o, op.k1 * op.vx + kr * (sxxdx + sxzdz));
-          stnt(a.vz + o, op.k1 * op.vz + kr * (sxzdx + szzdz));
+          st_out(a.vx + o, op.k1 * op.vx + kr * (sxxdx + sxzdz));
+          st_out(a.vz + o, op.k1 * op.vz + kr * (sxzdx + szzdz));
         } else {
           const float cv = op.cv, sv = op.sv;
-          stnt(a.vx + o, op.k1 * op.vx + kr * (cv * sxxdx - sv * sxxdz + sv * sxzdx + cv * sxzdz));
-          stnt(a.vz + o, op.k1 * op.vz + kr * (cv * sxzdx - sv * sxzdz + sv * szzdx + cv * szzdz));
+          st_out(a.vx + o, op.k1 * op.vx + kr * (cv * sxxdx - sv * sxxdz + sv * sxzdx + cv * sxzdz));
+          st_out(a.vz + o, op.k1 * op.vz + kr * (cv * sxzdx - sv * sxzdz + sv * szzdx + cv * szzdz));
         }
       }
 #pragma unroll
@@ -338,9 +338,9 @@ __global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
           const float t1 = k1 * op.s1 + k2 * (op.c11 * exx + op.c13 * ezz);
           const float t2 = k1 * op.s2 + k2 * (op.c13 * exx + op.c33 * ezz);
           const float t3 = k1 * op.s3 + k2 * op.c55 * exz;
-          stnt(a.txx + o, t1);
-          stnt(a.tzz + o, t2);
-          stnt(a.txz + o, t3);
+          st_out(a.txx + o, t1);
+          st_out(a.tzz + o, t2);
+          st_out(a.txz + o, t3);
           nxx = K2 * t1 + S2 * t2 + S2T * t3;
           nzz = S2 * t1 + K2 * t2 - S2T * t3;
           nxz = -0.5f * S2T * (t1 - t2) + (K2 - S2) * t3;
@@ -365,9 +365,9 @@ __global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
         }
       }
       if (zok) {
-        stnt(a.sxx + o, nxx);
-        stnt(a.szz + o, nzz);
-        stnt(a.sxz + o, nxz);
+        st_out(a.sxx + o, nxx);
+        st_out(a.szz + o, nzz);
+        st_out(a.sxz + o, nxz);
       }
 #pragma unroll
       for (int i = 0; i + 1 < PD; i++) {
