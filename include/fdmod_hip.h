@@ -98,6 +98,12 @@ const char *fdmod_strerror(int code);
 /* out[0..6] = Nz, Nx, ealThick (effective), izStart, izEnd, ixStart, ixEnd */
 int fdmod_dims(const fdmod_ctx *ctx, int *out7);
 
+/* Host-only stability check of staggered_grid_stable2d (L/fdmod/stable.c:217-242,
+ * called BT:472) on an unpadded nz*nx vp: FDMOD_OK or FDMOD_EUNSTABLE, with the
+ * limit in *dt_max.  Touches no device, so callers can validate before creating a
+ * context; fdmod_set_model performs the same check. */
+int fdmod_check_stability(const fdmod_params *p, const float *vp, double *dt_max);
+
 /* Unpadded nz*nx arrays in file order; theta in degrees.  vs and rho must be given
  * (the host programs apply vp2vs / vp2densityGardner, L/rockmod/modparconvert.c,
  * when the files are absent).  On FDMOD_EUNSTABLE *dt_max holds the limit. */

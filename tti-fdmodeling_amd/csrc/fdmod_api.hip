@@ -182,16 +182,24 @@ extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
 
 // staggered_grid_stable2d (L/fdmod/stable.c:217-242) on the padded pre-MEAL vp;
 // the padded maximum equals the maximum of the unpadded input.
-static double stability_limit(const fdmod_ctx *c, const float *vp)
+static double stability_limit(const fdmod_params &p, const float *vp)
 {
   static const float C[5] = {kC1, kC2, kC3, kC4, kC5};
   double vp_max = __FLT_MIN__;
-  size_t n = (size_t)c->p.nz * c->p.nx;
+  size_t n = (size_t)p.nz * p.nx;
   for (size_t i = 0; i < n; i++) vp_max = fmaxf(vp[i], (float)vp_max);
   double C_sum = 0.0f;
   for (int i = 0; i < kN; i += 2) C_sum += C[i] * pow(-1.0, 1.0 * i);
-  float dx = c->p.dx, dz = c->p.dz;
+  float dx = p.dx, dz = p.dz;
   return 1.0f / (vp_max * sqrt((1.0 / (dx * dx) + 1.0 / (dz * dz)) * C_sum));
+}
+
+extern "C" int fdmod_check_stability(const fdmod_params *p, const float *vp, double *dt_max)
+{
+  if (!p || !vp || p->nz < 1 || p->nx < 1 || !(p->dx > 0) || !(p->dz > 0)) return FDMOD_EINVAL;
+  double lim = stability_limit(*p, vp);
+  if (dt_max) *dt_max = lim;
+  return p->dt > lim ? FDMOD_EUNSTABLE : FDMOD_OK;
 }
 
 extern "C" int fdmod_set_model(fdmod_ctx *c, const float *vp, const float *vs, const float *rho, const float *eps,
@@ -201,7 +209,7 @@ extern "C" int fdmod_set_model(fdmod_ctx *c, const float *vp, const float *vs, c
   if (!vp || !vs || !rho || !eps || !delta || !theta) return fail(c, FDMOD_EINVAL, "model array is NULL");
   const fdmod_params &p = c->p;
   Grid &g = c->g;
-  double lim = stability_limit(c, vp);
+  double lim = stability_limit(p, vp);
   if (dt_max) *dt_max = lim;
   if (p.dt > lim)  // stable.c:236-241
     return fail(c, FDMOD_EUNSTABLE,

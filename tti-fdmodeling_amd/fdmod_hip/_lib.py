@@ -41,6 +41,7 @@ SIGNATURES = {
     "fdmod_last_error": (ct.c_char_p, [ct.c_void_p]),
     "fdmod_strerror": (ct.c_char_p, [ct.c_int]),
     "fdmod_dims": (ct.c_int, [ct.c_void_p, _ip]),
+    "fdmod_check_stability": (ct.c_int, [ct.POINTER(Params), _fp, _dp]),
     "fdmod_set_model": (ct.c_int, [ct.c_void_p] + [_fp] * 6 + [_dp]),
     "fdmod_set_source": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip, _ip, _fp]),
     "fdmod_set_receivers": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _ip, _ip]),
