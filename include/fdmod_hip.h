@@ -140,6 +140,33 @@ int fdmod_get_energy(fdmod_ctx *ctx, float *E, float *Ek, float *Ep);
 int fdmod_get_array(fdmod_ctx *ctx, int which, float *out);
 int fdmod_set_array(fdmod_ctx *ctx, int which, const float *in);
 
+/* ---- Batched shots (no counterpart in the reference, which runs one shot per program
+ * invocation, BT:239-243 shotID).  A batch holds nshots independent shots that share
+ * every parameter of *p except nx (shot s has nx[s] columns: its aperture crop).  Each
+ * shot is set up exactly like a single-shot context; the time loop then advances all
+ * shots together -- one launch per sweep covers every shot -- with K steps captured
+ * once in a hipGraph and replayed.  Snapshots and the energy series are single-shot
+ * features.  Results per shot are identical to running that shot alone. */
+typedef struct fdmod_batch fdmod_batch;
+int fdmod_batch_create(const fdmod_params *p, int nshots, const int *nx, fdmod_batch **out);
+void fdmod_batch_destroy(fdmod_batch *b);
+const char *fdmod_batch_last_error(const fdmod_batch *b);
+int fdmod_batch_set_model(fdmod_batch *b, int shot, const float *vp, const float *vs, const float *rho,
+                          const float *eps, const float *delta, const float *theta, double *dt_max);
+int fdmod_batch_set_source(fdmod_batch *b, int shot, int stype, int ntrace, const int *posx, const int *posz,
+                           const float *src);
+int fdmod_batch_set_receivers(fdmod_batch *b, int shot, int seistype, int mt, int ntrace, const int *posx,
+                              const int *posz);
+/* The shot's own context, for per-shot queries (fdmod_dims, fdmod_get_array, ...). */
+fdmod_ctx *fdmod_batch_shot(fdmod_batch *b, int shot);
+/* Autotuner: time the candidate segment lengths of both sweeps on this batch and keep
+ * the fastest (reps launches per candidate); resets the wavefields. */
+int fdmod_batch_tune(fdmod_batch *b, int reps, int *segLV, int *segLS);
+int fdmod_batch_set_tiles(fdmod_batch *b, int segLV, int segLS);
+int fdmod_batch_run(fdmod_batch *b, int k0, int count, double *seconds);
+int fdmod_batch_reset(fdmod_batch *b);
+int fdmod_batch_get_traces(fdmod_batch *b, int shot, int component, float *out);
+
 /* Measurement hook (no counterpart in the reference): launches the velocity sweep
  * reps times, then the stress sweep reps times, on the context's stream, and returns
  * the mean duration of one launch of each from HIP events recorded on that stream.

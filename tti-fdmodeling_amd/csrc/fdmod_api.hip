@@ -541,6 +541,7 @@ static int step(fdmod_ctx *c, int it, const SweepArgs &sa)
     r.isample = it / c->mt;
     r.want_v = c->seistype == 1 || c->seistype == 3;
     r.want_p = c->seistype == 2 || c->seistype == 3;
+    r.mt = c->mt;
     launch_receivers(c->stream, r);
   }
   if (c->energy) {
@@ -643,5 +644,352 @@ extern "C" int fdmod_time_sweeps(fdmod_ctx *c, int reps, double *ms_v, double *m
   HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   if (ms_s) *ms_s = (double)ms / reps;
   HIPCALL(c, hipGetLastError());
+  return FDMOD_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// Batched shots.  Each shot is a full modelling context (its own padded grid, MEAL,
+// Cij, source and receivers, set up exactly as a single shot); the time loop runs
+// all shots together: one launch per sweep covers every shot's tiles, and K steps
+// of launches are captured once into a hipGraph and replayed.  Kernels that depend
+// on the time index read the loop counter from device memory, so one captured
+// chunk serves the whole run.
+
+struct fdmod_batch {
+  fdmod_params p;
+  std::vector<fdmod_ctx *> shots;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int segLV = kDefaultSegLV, segLS = kDefaultSegLS;
+  bool prepared = false;
+  SweepArgs *d_sweep = nullptr;
+  int *d_startV = nullptr, *d_startS = nullptr, totalV = 0, totalS = 0;
+  SourceArgs *d_srcV = nullptr, *d_srcS = nullptr;
+  int srcBlocksV = 0, srcBlocksS = 0;
+  RecvArgs *d_rec = nullptr;
+  int maxnr = 0;
+  int *d_step = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int chunk = 32;
+  std::string err;
+};
+
+static int bfail(fdmod_batch *b, int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+
+#define BHIP(b, x)                                                                                  \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) return bfail((b), FDMOD_EDEVICE, "%s: %s", #x, hipGetErrorString(e_));   \
+  } while (0)
+
+static void batch_unprepare(fdmod_batch *b)
+{
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  dfree(b->d_sweep); dfree(b->d_startV); dfree(b->d_startS);
+  dfree(b->d_srcV); dfree(b->d_srcS); dfree(b->d_rec);
+  b->d_sweep = nullptr; b->d_startV = b->d_startS = nullptr;
+  b->d_srcV = b->d_srcS = nullptr; b->d_rec = nullptr;
+  b->prepared = false;
+}
+
+extern "C" void fdmod_batch_destroy(fdmod_batch *b)
+{
+  if (!b) return;
+  batch_unprepare(b);
+  dfree(b->d_step);
+  for (fdmod_ctx *c : b->shots) fdmod_destroy(c);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" const char *fdmod_batch_last_error(const fdmod_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+extern "C" int fdmod_batch_create(const fdmod_params *p, int nshots, const int *nx, fdmod_batch **out)
+{
+  if (!p || !out || nshots < 1 || !nx) return FDMOD_EINVAL;
+  *out = nullptr;
+  fdmod_batch *b = new fdmod_batch();
+  b->p = *p;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+      hipMalloc(&b->d_step, sizeof(int)) != hipSuccess) {
+    fdmod_batch_destroy(b);
+    return FDMOD_EDEVICE;
+  }
+  for (int s = 0; s < nshots; s++) {
+    fdmod_params q = *p;
+    q.nx = nx[s];
+    fdmod_ctx *c = nullptr;
+    int rc = fdmod_create(&q, &c);
+    if (rc != FDMOD_OK) {
+      fdmod_batch_destroy(b);
+      return rc;
+    }
+    b->shots.push_back(c);
+  }
+  *out = b;
+  return FDMOD_OK;
+}
+
+extern "C" fdmod_ctx *fdmod_batch_shot(fdmod_batch *b, int shot)
+{
+  if (!b || shot < 0 || shot >= (int)b->shots.size()) return nullptr;
+  b->prepared = false;  // the caller may change the shot through the single-shot API
+  return b->shots[shot];
+}
+
+static int batch_shot_rc(fdmod_batch *b, int shot, int rc)
+{
+  if (rc != FDMOD_OK) bfail(b, rc, "shot %d: %s", shot, b->shots[shot]->err.c_str());
+  b->prepared = false;
+  return rc;
+}
+
+extern "C" int fdmod_batch_set_model(fdmod_batch *b, int shot, const float *vp, const float *vs, const float *rho,
+                                     const float *eps, const float *delta, const float *theta, double *dt_max)
+{
+  if (!b || shot < 0 || shot >= (int)b->shots.size()) return FDMOD_EINVAL;
+  return batch_shot_rc(b, shot, fdmod_set_model(b->shots[shot], vp, vs, rho, eps, delta, theta, dt_max));
+}
+
+extern "C" int fdmod_batch_set_source(fdmod_batch *b, int shot, int stype, int ns, const int *posx, const int *posz,
+                                      const float *src)
+{
+  if (!b || shot < 0 || shot >= (int)b->shots.size()) return FDMOD_EINVAL;
+  return batch_shot_rc(b, shot, fdmod_set_source(b->shots[shot], stype, ns, posx, posz, src));
+}
+
+extern "C" int fdmod_batch_set_receivers(fdmod_batch *b, int shot, int seistype, int mt, int nr, const int *posx,
+                                         const int *posz)
+{
+  if (!b || shot < 0 || shot >= (int)b->shots.size()) return FDMOD_EINVAL;
+  return batch_shot_rc(b, shot, fdmod_set_receivers(b->shots[shot], seistype, mt, nr, posx, posz));
+}
+
+static SourceArgs shot_source(fdmod_ctx *c, const SweepArgs &sa, bool stress_phase)
+{
+  SourceArgs s;
+  memset(&s, 0, sizeof s);
+  s.g = c->g;
+  bool on = c->ns > 0 && (stress_phase ? c->stype == 1 : c->stype != 1);
+  if (!on) return s;  // ns = 0: nothing to do in this phase
+  s.isx = c->d_isx; s.isz = c->d_isz; s.src = c->d_src;
+  s.nt = c->p.nt; s.ns = c->ns; s.it = 0;
+  s.taps = c->d_taps; s.ntap = c->ntap; s.npass = c->npass; s.sequential = c->seq;
+  if (stress_phase) {
+    s.fa = sa.sxx;
+    s.fb = sa.szz;
+    if (c->p.form == FDMOD_FORM_TENSOR) {
+      s.fc = sa.txx;
+      s.fd = sa.tzz;
+    }
+  } else {
+    s.fa = c->stype == 6 ? sa.vz : sa.vx;
+    s.fb = (c->stype == 5 || c->stype == 6) ? nullptr : sa.vz;
+  }
+  return s;
+}
+
+static int batch_step(fdmod_batch *b, int off)
+{
+  const int n = (int)b->shots.size();
+  launch_vsweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startV, n, b->totalV);
+  if (b->srcBlocksV) launch_source_batch(b->stream, b->d_srcV, n, b->srcBlocksV, b->d_step, off, b->p.nt, b->p.mode);
+  launch_ssweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startS, n, b->totalS);
+  if (b->srcBlocksS) launch_source_batch(b->stream, b->d_srcS, n, b->srcBlocksS, b->d_step, off, b->p.nt, b->p.mode);
+  if (b->maxnr) launch_receivers_batch(b->stream, b->d_rec, n, b->maxnr, b->d_step, off, b->p.nt, b->p.mode);
+  return FDMOD_OK;
+}
+
+static int batch_prepare(fdmod_batch *b, bool with_graph)
+{
+  batch_unprepare(b);
+  const int n = (int)b->shots.size();
+  std::vector<SweepArgs> sw(n);
+  std::vector<SourceArgs> sv(n), ss(n);
+  std::vector<RecvArgs> rv(n);
+  std::vector<int> startV(n + 1, 0), startS(n + 1, 0);
+  b->srcBlocksV = b->srcBlocksS = b->maxnr = 0;
+  for (int s = 0; s < n; s++) {
+    fdmod_ctx *c = b->shots[s];
+    if (!c->model_set) return bfail(b, FDMOD_ESTATE, "shot %d has no model", s);
+    if (c->wsave > 0 || c->energy) return bfail(b, FDMOD_EINVAL, "shot %d: snapshots/energy are single-shot only", s);
+    c->g.segLV = b->segLV;
+    c->g.segLS = b->segLS;
+    sw[s] = sweep_args(c);
+    startV[s + 1] = startV[s] + sweep_tiles(c->g, b->segLV);
+    startS[s + 1] = startS[s] + sweep_tiles(c->g, b->segLS);
+    sv[s] = shot_source(c, sw[s], false);
+    ss[s] = shot_source(c, sw[s], true);
+    if (sv[s].ns) b->srcBlocksV = std::max(b->srcBlocksV, sv[s].sequential ? 1 : sv[s].ns);
+    if (ss[s].ns) b->srcBlocksS = std::max(b->srcBlocksS, ss[s].sequential ? 1 : ss[s].ns);
+    RecvArgs &r = rv[s];
+    memset(&r, 0, sizeof r);
+    if (c->seistype > 0 && c->nr > 0) {
+      r.nr = c->nr; r.ldz = c->g.ldz; r.gx = c->d_gx; r.gz = c->d_gz;
+      r.vx = sw[s].vx; r.vz = sw[s].vz; r.sxx = sw[s].sxx; r.szz = sw[s].szz;
+      r.out_vx = c->d_rvx; r.out_vz = c->d_rvz; r.out_p = c->d_rp;
+      r.want_v = c->seistype == 1 || c->seistype == 3;
+      r.want_p = c->seistype == 2 || c->seistype == 3;
+      r.mt = c->mt;
+      b->maxnr = std::max(b->maxnr, c->nr);
+    } else {
+      r.mt = 1;
+    }
+  }
+  b->totalV = startV[n];
+  b->totalS = startS[n];
+  BHIP(b, hipMalloc(&b->d_sweep, n * sizeof(SweepArgs)));
+  BHIP(b, hipMalloc(&b->d_startV, (n + 1) * sizeof(int)));
+  BHIP(b, hipMalloc(&b->d_startS, (n + 1) * sizeof(int)));
+  BHIP(b, hipMalloc(&b->d_srcV, n * sizeof(SourceArgs)));
+  BHIP(b, hipMalloc(&b->d_srcS, n * sizeof(SourceArgs)));
+  BHIP(b, hipMalloc(&b->d_rec, n * sizeof(RecvArgs)));
+  BHIP(b, hipMemcpy(b->d_sweep, sw.data(), n * sizeof(SweepArgs), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->d_startV, startV.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->d_startS, startS.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->d_srcV, sv.data(), n * sizeof(SourceArgs), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->d_srcS, ss.data(), n * sizeof(SourceArgs), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->d_rec, rv.data(), n * sizeof(RecvArgs), hipMemcpyHostToDevice));
+  for (fdmod_ctx *c : b->shots) BHIP(b, hipStreamSynchronize(c->stream));
+  if (with_graph) {
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < b->chunk; j++) batch_step(b, j);
+    launch_advance(b->stream, b->d_step, b->chunk);
+    hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+    if (e != hipSuccess) return bfail(b, FDMOD_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+    BHIP(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  }
+  b->prepared = true;
+  return FDMOD_OK;
+}
+
+static int batch_run_eager(fdmod_batch *b, int count)
+{
+  for (int j = 0; j < count; j++) batch_step(b, j);
+  launch_advance(b->stream, b->d_step, count);
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_batch_run(fdmod_batch *b, int k0, int count, double *seconds)
+{
+  if (!b) return FDMOD_EINVAL;
+  if (k0 < 0 || count < 0) return bfail(b, FDMOD_EINVAL, "bad iteration range");
+  count = std::max(0, std::min(count, b->p.nt - k0));
+  if (!b->prepared) {
+    int rc = batch_prepare(b, true);
+    if (rc) return rc;
+  }
+  BHIP(b, hipMemcpyAsync(b->d_step, &k0, sizeof(int), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipEventRecord(b->ev0, b->stream));
+  int done = 0;
+  while (count - done >= b->chunk) {
+    BHIP(b, hipGraphLaunch(b->gexec, b->stream));
+    done += b->chunk;
+  }
+  if (count > done) batch_run_eager(b, count - done);
+  BHIP(b, hipEventRecord(b->ev1, b->stream));
+  BHIP(b, hipEventSynchronize(b->ev1));
+  BHIP(b, hipGetLastError());
+  float ms = 0.0f;
+  BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  if (seconds) *seconds = ms * 1e-3;
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_batch_reset(fdmod_batch *b)
+{
+  if (!b) return FDMOD_EINVAL;
+  for (size_t s = 0; s < b->shots.size(); s++) {
+    int rc = fdmod_reset(b->shots[s]);
+    if (rc) return bfail(b, rc, "shot %zu: %s", s, b->shots[s]->err.c_str());
+  }
+  return FDMOD_OK;
+}
+
+extern "C" int fdmod_batch_get_traces(fdmod_batch *b, int shot, int comp, float *out)
+{
+  if (!b || shot < 0 || shot >= (int)b->shots.size()) return FDMOD_EINVAL;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  int rc = fdmod_get_traces(b->shots[shot], comp, out);
+  if (rc) return bfail(b, rc, "shot %d: %s", shot, b->shots[shot]->err.c_str());
+  return FDMOD_OK;
+}
+
+// Autotuner: times candidate segment lengths of both sweeps on this batch (each sweep
+// launched `reps` times per candidate, HIP events on the batch stream) and keeps the
+// fastest of each.  The wavefields advance during tuning and are reset afterwards.
+extern "C" int fdmod_batch_tune(fdmod_batch *b, int reps, int *segLV, int *segLS)
+{
+  if (!b || reps < 1) return FDMOD_EINVAL;
+  static const int candV[] = {22, 33, 44, 55, 66, 88};
+  static const int candS[] = {11, 22, 33, 44};
+  const int n = (int)b->shots.size();
+  auto time_one = [&](bool vsweep, int L, double &ms_out) -> int {
+    if (vsweep) b->segLV = L;
+    else b->segLS = L;
+    int rc = batch_prepare(b, false);
+    if (rc) return rc;
+    for (int w = 0; w < 2; w++) {  // warm-up launches
+      if (vsweep) launch_vsweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startV, n, b->totalV);
+      else launch_ssweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startS, n, b->totalS);
+    }
+    BHIP(b, hipEventRecord(b->ev0, b->stream));
+    for (int r = 0; r < reps; r++) {
+      if (vsweep) launch_vsweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startV, n, b->totalV);
+      else launch_ssweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startS, n, b->totalS);
+    }
+    BHIP(b, hipEventRecord(b->ev1, b->stream));
+    BHIP(b, hipEventSynchronize(b->ev1));
+    float ms = 0.0f;
+    BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    ms_out = ms / reps;
+    return FDMOD_OK;
+  };
+  double best = 1e30;
+  int bestV = b->segLV, bestS = b->segLS;
+  for (int L : candV) {
+    double ms;
+    int rc = time_one(true, L, ms);
+    if (rc) return rc;
+    if (ms < best) best = ms, bestV = L;
+  }
+  b->segLV = bestV;
+  best = 1e30;
+  for (int L : candS) {
+    double ms;
+    int rc = time_one(false, L, ms);
+    if (rc) return rc;
+    if (ms < best) best = ms, bestS = L;
+  }
+  b->segLS = bestS;
+  batch_unprepare(b);
+  if (segLV) *segLV = bestV;
+  if (segLS) *segLS = bestS;
+  return fdmod_batch_reset(b);
+}
+
+extern "C" int fdmod_batch_set_tiles(fdmod_batch *b, int segLV, int segLS)
+{
+  if (!b || segLV < 1 || segLS < 1 || segLV > 4096 || segLS > 4096) return FDMOD_EINVAL;
+  b->segLV = segLV;
+  b->segLS = segLS;
+  b->prepared = false;
   return FDMOD_OK;
 }

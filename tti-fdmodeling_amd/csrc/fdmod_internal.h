@@ -84,6 +84,7 @@ struct RecvArgs {
   const float *vx, *vz, *sxx, *szz;
   float *out_vx, *out_vz, *out_p;  // device [ntsample][nr]
   int isample, want_v, want_p;
+  int mt;                   // receiver sampling (batched launches derive isample from it)
 };
 
 // ---- launchers (kernels_setup.hip) ----
@@ -99,6 +100,15 @@ void launch_tile_flags(hipStream_t s, const Grid &g, const float *coef1, const f
 // ---- launchers (kernels_step.hip) ----
 void launch_vsweep(hipStream_t s, int form, const SweepArgs &a);
 void launch_ssweep(hipStream_t s, int form, const SweepArgs &a);
+int sweep_tiles(const Grid &g, int segL);
+// batched launches over many shots: device arrays of per-shot args, tile prefix sums
+void launch_vsweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total);
+void launch_ssweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total);
+void launch_source_batch(hipStream_t s, const SourceArgs *shots, int nshots, int maxblocks, const int *step, int off,
+                         int nt, int mode);
+void launch_receivers_batch(hipStream_t s, const RecvArgs *shots, int nshots, int maxnr, const int *step, int off,
+                            int nt, int mode);
+void launch_advance(hipStream_t s, int *step, int n);
 void launch_source(hipStream_t s, const SourceArgs &a);
 void launch_receivers(hipStream_t s, const RecvArgs &a);
 void launch_pressure(hipStream_t s, const float *sxx, const float *szz, float *p, int Nx, int ldz);

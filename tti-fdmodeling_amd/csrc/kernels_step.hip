@@ -149,13 +149,11 @@ __device__ __forceinline__ void load_vops(VOps &p, const SweepArgs &a, size_t o,
 // The loads of the staged column and of the per-point operands run PD iterations
 // ahead of their use.
 template <bool COORD, int PD>
-__global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
+__device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx)
 {
   __shared__ float ring[3][kRing][LZ];
   const Grid &g = a.g;
   const int segL = g.segLV;
-  int btz, btx;
-  if (!tile_of_block(g, segL, btz, btx)) return;
   const int lane = threadIdx.x;
   const int z0 = g.izStart + btz * TZ;
   const int xa = g.ixStart + btx * segL;
@@ -258,13 +256,11 @@ __device__ __forceinline__ void load_sops(SOps &p, const SweepArgs &a, size_t o,
 //   FORM_TENSOR TR:799-836 and the sigma <- R(tau) sweep TR:839-852, fused pointwise
 //   FORM_COORD  CR:804-845
 template <int FORM, int PD>
-__global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
+__device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx)
 {
   __shared__ float ring[2][kRing][LZ];
   const Grid &g = a.g;
   const int segL = g.segLS;
-  int btz, btx;
-  if (!tile_of_block(g, segL, btz, btx)) return;
   const int lane = threadIdx.x;
   const int z0 = g.izStart + btz * TZ;
   const int xa = g.ixStart + btx * segL;
@@ -389,11 +385,73 @@ __global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
 constexpr int kPipeV = FDMOD_PIPE_V;  // prefetch depth (iterations) of the velocity sweep
 constexpr int kPipeS = FDMOD_PIPE_S;  // and of the stress sweep
 
+// Single grid: blocks map to tiles of one shot.
+template <bool COORD, int PD>
+__global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
+{
+  int btz, btx;
+  if (!tile_of_block(a.g, a.g.segLV, btz, btx)) return;
+  vsweep_tile<COORD, PD>(a, btz, btx);
+}
+
+template <int FORM, int PD>
+__global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
+{
+  int btz, btx;
+  if (!tile_of_block(a.g, a.g.segLS, btz, btx)) return;
+  ssweep_tile<FORM, PD>(a, btz, btx);
+}
+
+// Batched grids: one launch covers the tiles of every shot.  start[s] is the first
+// global tile of shot s (prefix sum over shots, start[nshots] = total); the global
+// tile index is XCD-aware as in the single grid, then located by binary search.
+__device__ __forceinline__ bool batch_tile(const int *__restrict__ start, int nshots, int &shot, int &lt)
+{
+  const int total = start[nshots];
+  const int chunk = (total + kXCD - 1) / kXCD;
+  const int b = blockIdx.x;
+  const int t = (b % kXCD) * chunk + b / kXCD;
+  if (t >= total) return false;
+  int lo = 0, hi = nshots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  shot = lo;
+  lt = t - start[lo];
+  return true;
+}
+
+template <bool COORD, int PD>
+__global__ __launch_bounds__(64) void k_vsweep_batch(const SweepArgs *__restrict__ shots, const int *__restrict__ start,
+                                                     int nshots)
+{
+  int s, lt;
+  if (!batch_tile(start, nshots, s, lt)) return;
+  const SweepArgs a = shots[s];
+  const int btx = lt / a.g.tilesZ;
+  vsweep_tile<COORD, PD>(a, lt - btx * a.g.tilesZ, btx);
+}
+
+template <int FORM, int PD>
+__global__ __launch_bounds__(64) void k_ssweep_batch(const SweepArgs *__restrict__ shots, const int *__restrict__ start,
+                                                     int nshots)
+{
+  int s, lt;
+  if (!batch_tile(start, nshots, s, lt)) return;
+  const SweepArgs a = shots[s];
+  const int btx = lt / a.g.tilesZ;
+  ssweep_tile<FORM, PD>(a, lt - btx * a.g.tilesZ, btx);
+}
+
 static dim3 sweep_grid(const Grid &g, int segL)
 {
   const int n = g.tilesZ * ((g.ixEnd - g.ixStart + segL - 1) / segL);
   return dim3((n + kXCD - 1) / kXCD * kXCD);
 }
+
+int sweep_tiles(const Grid &g, int segL) { return g.tilesZ * ((g.ixEnd - g.ixStart + segL - 1) / segL); }
 
 void launch_vsweep(hipStream_t s, int form, const SweepArgs &a)
 {
@@ -410,20 +468,37 @@ void launch_ssweep(hipStream_t s, int form, const SweepArgs &a)
   else hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_BOND, kPipeS>), grid, dim3(64), 0, s, a);
 }
 
+void launch_vsweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total)
+{
+  dim3 grid((total + kXCD - 1) / kXCD * kXCD);
+  if (form == FDMOD_FORM_COORD) hipLaunchKernelGGL((k_vsweep_batch<true, kPipeV>), grid, dim3(64), 0, s, shots, start, nshots);
+  else hipLaunchKernelGGL((k_vsweep_batch<false, kPipeV>), grid, dim3(64), 0, s, shots, start, nshots);
+}
+
+void launch_ssweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total)
+{
+  dim3 grid((total + kXCD - 1) / kXCD * kXCD);
+  if (form == FDMOD_FORM_COORD)
+    hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_COORD, kPipeS>), grid, dim3(64), 0, s, shots, start, nshots);
+  else if (form == FDMOD_FORM_TENSOR)
+    hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_TENSOR, kPipeS>), grid, dim3(64), 0, s, shots, start, nshots);
+  else hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_BOND, kPipeS>), grid, dim3(64), 0, s, shots, start, nshots);
+}
+
 // Source injection (BT:706-834 on velocity, BT:893-914 on stress).  Each pass is a
 // table of taps; one thread per tap adds ca*src[it] to fa (and fc) and cb*src[it]
 // to fb (and fd) at the tap's cell when it lies inside the grid (the reference's
 // MAX/MIN clipping).  Passes run in order (stype 4: divergence, then curl), and
 // within a pass the sources are walked in order inside one block when their
 // footprints may overlap, so every cell sees the additions in the reference's order.
-__global__ void k_source(SourceArgs a)
+__device__ __forceinline__ void source_body(const SourceArgs &a, int it, int blk)
 {
   const Grid &g = a.g;
-  const int first = a.sequential ? 0 : blockIdx.x;
-  const int last = a.sequential ? a.ns : blockIdx.x + 1;
+  const int first = a.sequential ? 0 : blk;
+  const int last = a.sequential ? a.ns : min(blk + 1, a.ns);
   for (int p = 0; p < a.npass; p++) {
     for (int is = first; is < last; is++) {
-      const float sv = a.src[(size_t)is * a.nt + a.it];
+      const float sv = a.src[(size_t)is * a.nt + it];
       for (int t = threadIdx.x; t < a.ntap; t += blockDim.x) {
         const SrcTap tp = a.taps[p * a.ntap + t];
         const int ix = a.isx[is] + tp.ox, iz = a.isz[is] + tp.oz;
@@ -443,20 +518,42 @@ __global__ void k_source(SourceArgs a)
   }
 }
 
+__global__ void k_source(SourceArgs a) { source_body(a, a.it, blockIdx.x); }
+
 void launch_source(hipStream_t s, const SourceArgs &a)
 {
   dim3 grid(a.sequential ? 1 : a.ns);
   hipLaunchKernelGGL(k_source, grid, dim3(128), 0, s, a);
 }
 
+// Time index of loop iteration k = *step + off (BT:651): it = k or nt-1-k.
+__device__ __forceinline__ int step_it(const int *step, int off, int nt, int mode)
+{
+  const int k = *step + off;
+  return mode == 1 ? k : nt - 1 - k;
+}
+
+// Batched sources: blockIdx.y = shot; shots without a source in this phase have ns = 0.
+__global__ void k_source_batch(const SourceArgs *__restrict__ shots, const int *step, int off, int nt, int mode)
+{
+  const SourceArgs a = shots[blockIdx.y];
+  if (a.ns == 0 || (!a.sequential && (int)blockIdx.x >= a.ns) || (a.sequential && blockIdx.x > 0)) return;
+  source_body(a, step_it(step, off, nt, mode), blockIdx.x);
+}
+
+void launch_source_batch(hipStream_t s, const SourceArgs *shots, int nshots, int maxblocks, const int *step, int off,
+                         int nt, int mode)
+{
+  hipLaunchKernelGGL(k_source_batch, dim3(maxblocks, nshots), dim3(128), 0, s, shots, step, off, nt, mode);
+}
+
 // elastic2dReceiver (L/fdmod/isoelastic.c:103-144): nearest-point gathers into a
 // device buffer laid out [sample][trace].
-__global__ void k_receivers(RecvArgs a)
+__device__ __forceinline__ void receiver_body(const RecvArgs &a, int isample, int r)
 {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.nr) return;
   size_t o = (size_t)a.gx[r] * a.ldz + a.gz[r];
-  size_t d = (size_t)a.isample * a.nr + r;
+  size_t d = (size_t)isample * a.nr + r;
   if (a.want_v) {
     a.out_vx[d] = a.vx[o];
     a.out_vz[d] = a.vz[o];
@@ -464,10 +561,36 @@ __global__ void k_receivers(RecvArgs a)
   if (a.want_p) a.out_p[d] = 0.5f * (a.sxx[o] + a.szz[o]);
 }
 
+__global__ void k_receivers(RecvArgs a) { receiver_body(a, a.isample, blockIdx.x * blockDim.x + threadIdx.x); }
+
 void launch_receivers(hipStream_t s, const RecvArgs &a)
 {
   hipLaunchKernelGGL(k_receivers, dim3((a.nr + 255) / 256), dim3(256), 0, s, a);
 }
+
+// Batched receivers: blockIdx.y = shot; records when it % mt == 0 (isoelastic.c:107).
+__global__ void k_receivers_batch(const RecvArgs *__restrict__ shots, const int *step, int off, int nt, int mode)
+{
+  const RecvArgs a = shots[blockIdx.y];
+  const int it = step_it(step, off, nt, mode);
+  if (a.nr == 0 || it % a.mt != 0) return;
+  receiver_body(a, it / a.mt, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+void launch_receivers_batch(hipStream_t s, const RecvArgs *shots, int nshots, int maxnr, const int *step, int off,
+                            int nt, int mode)
+{
+  hipLaunchKernelGGL(k_receivers_batch, dim3((maxnr + 255) / 256, nshots), dim3(256), 0, s, shots, step, off, nt,
+                     mode);
+}
+
+// Advance the device-side loop counter by n iterations (last node of a captured chunk).
+__global__ void k_advance(int *step, int n)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += n;
+}
+
+void launch_advance(hipStream_t s, int *step, int n) { hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, step, n); }
 
 // Pressure snapshot field p = 0.5*(sxx + szz) (isoelastic.c:259-266).
 __global__ void k_pressure(const float *sxx, const float *szz, float *p, size_t n)

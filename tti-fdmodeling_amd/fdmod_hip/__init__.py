@@ -11,7 +11,7 @@ import numpy as np
 
 from ._lib import AID, FORMS, FdmodError, Params, SNAPSHOT_FN, _f32, _ptr, load
 
-__all__ = ["Modeling", "FdmodError", "FORMS", "load"]
+__all__ = ["Modeling", "Batch", "FdmodError", "FORMS", "load"]
 
 
 class Modeling:
@@ -132,3 +132,82 @@ class Modeling:
         v, s = ct.c_double(0), ct.c_double(0)
         self._check(self.L.fdmod_time_sweeps(self.c, reps, ct.byref(v), ct.byref(s)))
         return v.value, s.value
+
+
+class Batch:
+    """Many independent shots advanced together (one launch per sweep over all shots,
+    time loop captured in a hipGraph).  Shot s has its own model width nx[s]; all other
+    parameters are shared.  Per-shot results equal those of a single-shot Modeling."""
+
+    def __init__(self, form, nz, nxs, nt, dx, dz, dt, ealThick=30, eal_alpha=-1.0, freesurface=0, fpeak=10.0,
+                 scalar=1.0, mode=1):
+        self.L = load()
+        p = Params(FORMS[form] if isinstance(form, str) else int(form), nz, int(nxs[0]), nt, dx, dz, dt, ealThick,
+                   eal_alpha, freesurface, fpeak, scalar, mode)
+        nx = np.ascontiguousarray(nxs, dtype=np.int32)
+        h = ct.c_void_p()
+        rc = self.L.fdmod_batch_create(ct.byref(p), len(nx), _ptr(nx, ct.c_int), ct.byref(h))
+        if rc != 0:
+            raise FdmodError(rc, self.L.fdmod_strerror(rc).decode())
+        self.b = h
+        self.nz, self.nxs, self.nt = nz, list(nx), nt
+        self.nr = [0] * len(nx)
+        self.ntsample = [0] * len(nx)
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.L.fdmod_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FdmodError(rc, self.L.fdmod_batch_last_error(self.b).decode())
+
+    def set_model(self, shot, vp, vs, rho, eps, delta, theta):
+        arrs = [_f32(a).reshape(-1) for a in (vp, vs, rho, eps, delta, theta)]
+        for a in arrs:
+            if a.size != self.nz * self.nxs[shot]:
+                raise ValueError("model arrays must have nz*nx[shot] samples")
+        dtm = ct.c_double(0)
+        self._check(self.L.fdmod_batch_set_model(self.b, shot, *[_ptr(a, ct.c_float) for a in arrs], ct.byref(dtm)))
+
+    def set_source(self, shot, stype, posx, posz, src):
+        posx = np.ascontiguousarray(posx, dtype=np.int32)
+        posz = np.ascontiguousarray(posz, dtype=np.int32)
+        src = _f32(src).reshape(len(posx), self.nt)
+        self._check(self.L.fdmod_batch_set_source(self.b, shot, stype, len(posx), _ptr(posx, ct.c_int),
+                                                  _ptr(posz, ct.c_int), _ptr(src, ct.c_float)))
+
+    def set_receivers(self, shot, seistype, mt, posx, posz):
+        posx = np.ascontiguousarray(posx, dtype=np.int32)
+        posz = np.ascontiguousarray(posz, dtype=np.int32)
+        self._check(self.L.fdmod_batch_set_receivers(self.b, shot, seistype, mt, len(posx), _ptr(posx, ct.c_int),
+                                                     _ptr(posz, ct.c_int)))
+        self.nr[shot] = len(posx)
+        self.ntsample[shot] = (self.nt - 1) // mt + 1
+
+    def tune(self, reps=10):
+        v, s = ct.c_int(0), ct.c_int(0)
+        self._check(self.L.fdmod_batch_tune(self.b, reps, ct.byref(v), ct.byref(s)))
+        return v.value, s.value
+
+    def set_tiles(self, segLV, segLS):
+        self._check(self.L.fdmod_batch_set_tiles(self.b, segLV, segLS))
+
+    def run(self, k0=0, count=None):
+        if count is None:
+            count = self.nt - k0
+        sec = ct.c_double(0)
+        self._check(self.L.fdmod_batch_run(self.b, k0, count, ct.byref(sec)))
+        return sec.value
+
+    def reset(self):
+        self._check(self.L.fdmod_batch_reset(self.b))
+
+    def traces(self, shot, comp):
+        out = np.empty(self.nr[shot] * self.ntsample[shot], dtype=np.float32)
+        self._check(self.L.fdmod_batch_get_traces(self.b, shot, comp, _ptr(out, ct.c_float)))
+        return out.reshape(self.nr[shot], self.ntsample[shot])

@@ -54,6 +54,18 @@ SIGNATURES = {
     "fdmod_get_array": (ct.c_int, [ct.c_void_p, ct.c_int, _fp]),
     "fdmod_set_array": (ct.c_int, [ct.c_void_p, ct.c_int, _fp]),
     "fdmod_time_sweeps": (ct.c_int, [ct.c_void_p, ct.c_int, _dp, _dp]),
+    "fdmod_batch_create": (ct.c_int, [ct.POINTER(Params), ct.c_int, _ip, ct.POINTER(ct.c_void_p)]),
+    "fdmod_batch_destroy": (None, [ct.c_void_p]),
+    "fdmod_batch_last_error": (ct.c_char_p, [ct.c_void_p]),
+    "fdmod_batch_set_model": (ct.c_int, [ct.c_void_p, ct.c_int] + [_fp] * 6 + [_dp]),
+    "fdmod_batch_set_source": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _ip, _ip, _fp]),
+    "fdmod_batch_set_receivers": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _ip, _ip]),
+    "fdmod_batch_shot": (ct.c_void_p, [ct.c_void_p, ct.c_int]),
+    "fdmod_batch_tune": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, _ip]),
+    "fdmod_batch_set_tiles": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int]),
+    "fdmod_batch_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _dp]),
+    "fdmod_batch_reset": (ct.c_int, [ct.c_void_p]),
+    "fdmod_batch_get_traces": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _fp]),
 }
 
 _LIB = None
