@@ -115,6 +115,64 @@ def cpu_baseline(cfg, m, src, form, steps, g_fields):
     return res, parity
 
 
+def batched_vs_loop(form, m, cfg, steps, nshots=64, seed=3):
+    """64 shots on the full model, each on its own aperture crop of 1500-4000 columns
+    (seeded, centred on the source and clamped to the model), receivers in every crop
+    column: all shots advanced together (batched launches, captured hipGraph loop,
+    autotuned tiles) against the same shots run one after another on the single-shot
+    path.  Both time `steps` time steps per shot; inputs resident before timing."""
+    from fdmod_hip import Batch, Modeling, synth
+
+    rng = np.random.default_rng(seed)
+    nx_full = m["vp"].shape[0]
+    centres = np.linspace(300, nx_full - 300, nshots).astype(int)
+    widths = rng.integers(1500, 4001, size=nshots)
+    crops = []
+    for c, w in zip(centres, widths):
+        x0 = int(np.clip(c - w // 2, 0, nx_full - w))
+        crops.append((x0, int(w), int(c - x0)))
+    src = synth.ricker(cfg["fpeak"], cfg["dt"], steps)
+    kw = dict(ealThick=cfg["ealThick"], freesurface=cfg["freesurface"], fpeak=cfg["fpeak"])
+    b = Batch(form, cfg["nz"], [w for _, w, _ in crops], steps, cfg["dx"], cfg["dz"], cfg["dt"], **kw)
+    pts = 0
+    for s, (x0, w, sx) in enumerate(crops):
+        mc = synth.crop(m, x0, w)
+        b.set_model(s, mc["vp"], mc["vs"], mc["rho"], mc["eps"], mc["delta"], mc["theta"])
+        b.set_source(s, 1, [sx], [cfg["sz"]], src[None])
+        b.set_receivers(s, 2, 1, np.arange(w), np.full(w, cfg["rz"]))
+        izs = cfg["ealThick"] + 5 if cfg["freesurface"] else 5
+        pts += (w + 2 * (cfg["ealThick"] + 5) - 10) * (cfg["nz"] + 2 * (cfg["ealThick"] + 5) - 5 - izs)
+    log(f"bench: batched {nshots} shots, autotuning tiles")
+    tiles = b.tune(5)
+    b.run(0, min(32, steps))
+    b.reset()
+    gpu_sync()
+    t0 = time.perf_counter()
+    b.run(0, steps)
+    gpu_sync()
+    wb = time.perf_counter() - t0
+    b.close()
+    log(f"bench: the same {nshots} shots one after another")
+    wl = 0.0
+    for s, (x0, w, sx) in enumerate(crops):
+        mc = synth.crop(m, x0, w)
+        g = Modeling(form, cfg["nz"], w, steps, cfg["dx"], cfg["dz"], cfg["dt"], **kw)
+        g.set_model(mc["vp"], mc["vs"], mc["rho"], mc["eps"], mc["delta"], mc["theta"])
+        g.set_source(1, [sx], [cfg["sz"]], src[None])
+        g.set_receivers(2, 1, np.arange(w), np.full(w, cfg["rz"]))
+        gpu_sync()
+        t0 = time.perf_counter()
+        g.run(0, steps)
+        gpu_sync()
+        wl += time.perf_counter() - t0
+        g.close()
+    return {"workload": f"{nshots} shots, {form} form, aperture crops 1500-4000 columns of the 1126x7872 model "
+                        f"(seed {seed}), 1126 deep, free surface, receivers in every crop column",
+            "steps": steps, "updated_points_per_step": pts, "tiles_segLV_segLS": list(tiles),
+            "gpts_batched": round(pts * steps / wb / 1e9, 3), "gpts_single_shot_loop": round(pts * steps / wl / 1e9, 3),
+            "speedup": round(wl / wb, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +182,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU shot sharding is a design note only",
@@ -156,6 +215,10 @@ def main():
         with open(tf) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
 
+    batched = None
+    if args.batch_steps > 0:
+        batched = batched_vs_loop(args.form, m, cfg, args.batch_steps)
+
     forms = {}
     if not args.no_forms:
         for fm in ("bond", "coord", "tensor"):
@@ -186,10 +249,10 @@ def main():
             gc = make(__import__("fdmod_hip").Modeling, "coord", cfg, m, src)
             gc.run(0, n2)
             fc = {k: gc.get(k) for k in ("vx", "vz", "sxx", "szz", "sxz")}
-            finite = all(bool(np.isfinite(v).all()) for v in fc.values())
+            coord_finite = all(bool(np.isfinite(v).all()) for v in fc.values())
             gc.close()
             _, pc = cpu_baseline(cfg, m, src, "coord", n2, fc)
-            pc["finite"] = finite
+            pc["finite"] = coord_finite
             parity["coord"] = pc
 
     out = {
@@ -212,6 +275,7 @@ def main():
         "traces_finite": finite,
         "roofline": roofline,
         "forms_gpts": forms,
+        "batched": batched,
         "cpu_baseline": cpu,
         "parity_full_size": parity,
     }
