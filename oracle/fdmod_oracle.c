@@ -662,3 +662,6 @@ int PFX(set_threads)(int n)
   return 1;
 #endif
 }
+
+/* ---------------- adjoint and gradient ---------------- */
+#include "fdmod_oracle_adj.inc"

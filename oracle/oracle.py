@@ -59,6 +59,9 @@ def _lib(prec):
         "get_traces": (ct.c_int, [ct.c_void_p, ct.c_int, dp]),
         "get_energy": (ct.c_int, [ct.c_void_p, dp, dp, dp]),
         "set_threads": (ct.c_int, [ct.c_int]),
+        "gradient": (ct.c_int, [ct.c_void_p, fp, fp, fp, dp, dp, dp]),
+        "adjoint_source": (ct.c_int, [ct.c_void_p, dp, dp, dp, dp]),
+        "set_model_frozen": (ct.c_int, [ct.c_void_p] + [fp] * 6),
     }
     fns = {}
     for k, (res, args) in sig.items():
@@ -164,6 +167,38 @@ class Oracle:
         rc = self.L["get_traces"](self.c, comp, _ptr(out, ct.c_double))
         assert rc == 0
         return out.reshape(self.nr, self.ntsample)
+
+    def set_model_frozen(self, vp, vs, rho, eps, delta, theta):
+        """Replace the model keeping the absorbing layers of the last set_model."""
+        arrs = [_f32(a).reshape(-1) for a in (vp, vs, rho, eps, delta, theta)]
+        for a in arrs:
+            assert a.size == self.nz * self.nx
+        self.L["set_model_frozen"](self.c, *[_ptr(a, ct.c_float) for a in arrs])
+
+    def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, want_gc=False):
+        """J = 1/2 sum (d - obs)^2 and dJ/d(vp, eps, delta, theta[deg]) as (4, nx, nz),
+        runs the forward from rest (Bond form).  want_gc also returns dJ/dc for the
+        six padded stiffness arrays, (6, Nx, Nz)."""
+        J = ct.c_double(0)
+        g = np.zeros(4 * self.nx * self.nz)
+        gc = np.zeros(6 * self.Nx * self.Nz) if want_gc else None
+        obs = [None if o is None else _f32(o).reshape(-1) for o in (obs_vx, obs_vz, obs_p)]
+        rc = self.L["gradient"](self.c, *[None if o is None else _ptr(o, ct.c_float) for o in obs], ct.byref(J),
+                                _ptr(g, ct.c_double), None if gc is None else _ptr(gc, ct.c_double))
+        if rc != 0:
+            raise ValueError(self.L["last_error"](self.c).decode())
+        g = g.reshape(4, self.nx, self.nz)
+        return (J.value, g, gc.reshape(6, self.Nx, self.Nz)) if want_gc else (J.value, g)
+
+    def adjoint_source(self, y_vx=None, y_vz=None, y_p=None, ns=1):
+        """F^T y: the adjoint of the source time functions, (ns, nt)."""
+        ys = [None if y is None else np.ascontiguousarray(y, dtype=np.float64).reshape(-1) for y in (y_vx, y_vz, y_p)]
+        out = np.zeros(ns * self.nt)
+        rc = self.L["adjoint_source"](self.c, *[None if y is None else _ptr(y, ct.c_double) for y in ys],
+                                      _ptr(out, ct.c_double))
+        if rc != 0:
+            raise ValueError(self.L["last_error"](self.c).decode())
+        return out.reshape(ns, self.nt)
 
     def energy(self):
         E = np.empty(self.nt)
