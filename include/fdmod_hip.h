@@ -167,6 +167,34 @@ int fdmod_batch_run(fdmod_batch *b, int k0, int count, double *seconds);
 int fdmod_batch_reset(fdmod_batch *b);
 int fdmod_batch_get_traces(fdmod_batch *b, int shot, int component, float *out);
 
+/* ---- Adjoint and misfit gradient (no counterpart in the reference, which only runs
+ * the forward loop; Bond form).  The adjoint is the exact transpose of the loop
+ * iteration BT:651-954 as implemented by fdmod_run (sweeps, velocity/stress sources,
+ * free-surface imaging, receiver sampling), with the absorbing layers (MEAL D, Beta,
+ * coef1, coef2 of the last fdmod_set_model) held fixed.
+ *
+ * fdmod_gradient: J = 1/2 sum_c sum_r sum_s (d_c[r][s] - obs_c[r][s])^2 over the
+ * recorded components c whose obs_c is given (vx and vz go together), with d from a
+ * run from rest; grad receives dJ/d(vp, epsilon, delta, theta in degrees) as
+ * [4][nx][nz] doubles in the model-file layout.  The forward wavefield is rebuilt by
+ * checkpointed recomputation with ncheckpoints stored states (<= 0: ceil(sqrt(nt))).
+ * Imaging sums accumulate in FP64.  Afterwards the traces are those of the run from
+ * rest; the wavefield is not.  vs and rho are not differentiated. */
+int fdmod_gradient(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, const float *obs_p, int ncheckpoints,
+                   double *misfit, double *grad);
+/* Replace the model keeping the absorbing layers of the last fdmod_set_model (the
+ * model the gradient is defined with); same array conventions as fdmod_set_model. */
+int fdmod_update_model(fdmod_ctx *ctx, const float *vp, const float *vs, const float *rho, const float *eps,
+                       const float *delta, const float *theta);
+/* F^T y, the adjoint of the modelling operator F (source time functions -> traces)
+ * applied to data-space series y ([ntrace][ntsample] per component, NULL = absent):
+ * out [ntrace_src][nt] doubles. */
+int fdmod_adjoint_source(fdmod_ctx *ctx, const float *y_vx, const float *y_vz, const float *y_p, double *out);
+/* Dot-product test on the device: *lhs = <F s, y> from a run from rest with the
+ * current source s, *rhs = <s, F^T y>; both reductions in FP64 on the GPU. */
+int fdmod_adjoint_dot(fdmod_ctx *ctx, const float *y_vx, const float *y_vz, const float *y_p, double *lhs,
+                      double *rhs);
+
 /* Measurement hook (no counterpart in the reference): launches the velocity sweep
  * reps times, then the stress sweep reps times, on the context's stream, and returns
  * the mean duration of one launch of each from HIP events recorded on that stream.

@@ -1,0 +1,152 @@
+"""GPU adjoint and misfit gradient (Bond form) against the FP64 oracle adjoint,
+the device dot-product test, finite differences of the GPU misfit along 16 random
+directions, checkpoint-count invariance and the torch.autograd.Function wrapper.
+
+Tolerances: the GPU runs FP32 stencils (FP64 reductions and imaging sums), the
+oracle runs FP64 throughout.  Adjoint sources and gradients agree to a relative L2
+of 1e-4; the dot product closes to 1e-5 (FP32 propagation, FP64 sums).
+"""
+import numpy as np
+import pytest
+
+from common import rel_l2
+from fdmod_hip import Modeling, synth
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+NZ, NX, NT, DT = 40, 60, 300, 1e-3
+
+
+def _model(seed=3):
+    return synth.model_small(nz=NZ, nx=NX, seed=seed)
+
+
+def _mk(cls, model, fs=0, stype=1, seis=2, mode=1, mt=1, **kw):
+    o = cls("bond", NZ, NX, NT, 10.0, 10.0, DT, ealThick=10, freesurface=fs, fpeak=15.0, mode=mode, **kw)
+    o.set_model(model["vp"], model["vs"], model["rho"], model["eps"], model["delta"], model["theta"])
+    o.set_source(stype, [30], [6], synth.ricker(15.0, DT, NT)[None])
+    rx = np.arange(0, NX, 2)
+    o.set_receivers(seis, mt, rx, np.full(len(rx), 3))
+    return o
+
+
+def _use(seis):
+    return {1: seis in (1, 3), 3: seis in (1, 3), 4: seis in (2, 3)}
+
+
+@pytest.mark.parametrize("fs,stype,seis,mode,mt", [(0, 1, 2, 1, 1), (1, 1, 3, 1, 2), (0, 2, 3, 2, 1),
+                                                   (1, 4, 1, 1, 1), (1, 7, 3, 1, 3), (0, 8, 2, 1, 1)])
+def test_dot_product_and_adjoint_source(fs, stype, seis, mode, mt):
+    print("dot", fs, stype, seis, mode, mt, flush=True)
+    rng = np.random.default_rng(stype)
+    g = _mk(Modeling, _model(), fs, stype, seis, mode, mt)
+    use = _use(seis)
+    shape = (g.nr, g.ntsample)
+    y = {c: rng.standard_normal(shape).astype(np.float32) for c in (1, 3, 4)}
+    ys = [y[c] if use[c] else None for c in (1, 3, 4)]
+    lhs, rhs = g.adjoint_dot(*ys)
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs), (lhs, rhs)
+    a_gpu = g.adjoint_source(*ys)
+    o = _mk(Oracle, _model(), fs, stype, seis, mode, mt, prec="f64")
+    a_ref = o.adjoint_source(*[None if v is None else v.astype(np.float64) for v in ys])
+    assert rel_l2(a_gpu, a_ref) <= 1e-4
+
+
+def _obs(fs, seis, mode=1, mt=1):
+    m = _model()
+    true = {k: v.copy() for k, v in m.items()}
+    true["vp"] = (true["vp"] * 1.03).astype(np.float32)
+    true["theta"] = (true["theta"] + 4).astype(np.float32)
+    ot = _mk(Oracle, true, fs, seis=seis, mode=mode, mt=mt, prec="f64")
+    ot.run()
+    use = _use(seis)
+    return m, [ot.traces(c).astype(np.float32) if use[c] else None for c in (1, 3, 4)]
+
+
+@pytest.mark.parametrize("fs,seis,mode,mt", [(0, 2, 1, 1), (1, 3, 1, 2), (1, 1, 2, 1)])
+def test_gradient_matches_fp64_oracle(fs, seis, mode, mt):
+    print("gradient", fs, seis, mode, mt, flush=True)
+    m, obs = _obs(fs, seis, mode, mt)
+    g = _mk(Modeling, m, fs, seis=seis, mode=mode, mt=mt)
+    J, grad = g.gradient(*obs)
+    o = _mk(Oracle, m, fs, seis=seis, mode=mode, mt=mt, prec="f64")
+    Jr, gr = o.gradient(*obs)
+    assert abs(J - Jr) <= 1e-5 * Jr
+    for i in range(4):
+        assert rel_l2(grad[i], gr[i]) <= 1e-4, i
+    # the traces left in the context are the run from rest
+    assert np.array_equal(g.traces(4 if seis != 1 else 1), g.traces(4 if seis != 1 else 1))
+
+
+def test_gradient_independent_of_checkpoint_count():
+    m, obs = _obs(1, 2)
+    g = _mk(Modeling, m, 1, seis=2)
+    ref = g.gradient(*obs, ncheckpoints=0)
+    for ncp in (1, 7, NT):
+        J, grad = g.gradient(*obs, ncheckpoints=ncp)
+        assert J == ref[0] and np.array_equal(grad, ref[1]), ncp
+
+
+def test_gradient_finite_differences_16_directions():
+    m, obs = _obs(0, 2)
+    g = _mk(Modeling, m, 0, seis=2)
+    J0, grad = g.gradient(*obs)
+
+    def J_of(model):
+        g.update_model(model["vp"], model["vs"], model["rho"], model["eps"], model["delta"], model["theta"])
+        g.run()
+        r = g.traces(4).astype(np.float64) - obs[2]
+        return 0.5 * float((r * r).sum())
+
+    assert abs(J_of(m) - J0) <= 1e-6 * J0
+    rng = np.random.default_rng(42)
+    keys = [("vp", 0, 2.0), ("eps", 1, 2e-3), ("delta", 2, 2e-3), ("theta", 3, 0.5)]
+    for d in range(16):
+        key, gi, sc = keys[d % 4]
+        dm = rng.standard_normal(m[key].shape)
+        mp, mm = dict(m), dict(m)
+        mp[key] = (m[key] + sc * dm).astype(np.float32)
+        mm[key] = (m[key] - sc * dm).astype(np.float32)
+        step = mp[key].astype(np.float64) - mm[key].astype(np.float64)
+        fd = J_of(mp) - J_of(mm)
+        an = float((grad[gi] * step).sum())
+        assert abs(fd - an) <= 5e-3 * abs(fd), (d, key, fd, an)
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    from fdmod_hip.torch_misfit import misfit
+
+    m, obs = _obs(0, 2)
+    g = _mk(Modeling, m, 0, seis=2)
+    J_ref, grad_ref = g.gradient(*obs)
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    ts = {k: torch.tensor(m[k], device=dev, requires_grad=True) for k in ("vp", "eps", "delta", "theta")}
+    J = misfit(ts["vp"], ts["eps"], ts["delta"], ts["theta"], g, m["vs"], m["rho"], {"p": obs[2]})
+    (3.0 * J).backward()
+    assert abs(J.item() - J_ref) <= 1e-12 * J_ref
+    for i, k in enumerate(("vp", "eps", "delta", "theta")):
+        got = ts[k].grad.double().cpu().numpy()
+        assert ts[k].grad.device.type == dev
+        assert rel_l2(got, 3.0 * grad_ref[i]) <= 1e-6, k
+    with torch.no_grad():
+        J2 = misfit(ts["vp"], ts["eps"], ts["delta"], ts["theta"], g, m["vs"], m["rho"], {"p": obs[2]})
+    assert abs(J2.item() - J_ref) <= 1e-6 * J_ref
+
+
+def test_dot_product_at_gradient_workload_size():
+    # the adjoint-gradient workload's grid: a 1126 x 2000 crop, nt = 4000, dt = 0.8 ms
+    cfg = synth.CONFIGS["large"]
+    m = synth.crop(synth.model_large(), 2000, 2000)
+    nt = 4000
+    print("dot at 1126x2000, nt=4000", flush=True)
+    g = Modeling("bond", cfg["nz"], 2000, nt, cfg["dx"], cfg["dz"], cfg["dt"], ealThick=cfg["ealThick"],
+                 freesurface=0, fpeak=cfg["fpeak"])
+    g.set_model(m["vp"], m["vs"], m["rho"], m["eps"], m["delta"], m["theta"])
+    g.set_source(1, [1000], [cfg["sz"]], synth.ricker(cfg["fpeak"], cfg["dt"], nt)[None])
+    rx = np.arange(2000)
+    g.set_receivers(2, 1, rx, np.full(2000, cfg["rz"]))
+    y = np.random.default_rng(5).standard_normal((2000, nt)).astype(np.float32)
+    lhs, rhs = g.adjoint_dot(None, None, y)
+    assert np.isfinite(lhs) and abs(lhs - rhs) <= 1e-4 * abs(lhs), (lhs, rhs)

@@ -11,41 +11,11 @@
 #include <string>
 #include <vector>
 
-#include "fdmod_internal.h"
+#include "fdmod_ctx.h"
 
 using namespace fdm;
 
-struct fdmod_ctx {
-  fdmod_params p;
-  Grid g;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DevArrays d;
-  float *pool = nullptr;
-  uint8_t *flags = nullptr;
-  bool model_set = false;
-  // source
-  int stype = 0, ns = 0, npass = 0, ntap = 0, seq = 1;
-  int *d_isx = nullptr, *d_isz = nullptr;
-  float *d_src = nullptr;
-  SrcTap *d_taps = nullptr;
-  // receivers
-  int seistype = 0, mt = 1, nr = 0, ntsample = 0;
-  int *d_gx = nullptr, *d_gz = nullptr;
-  float *d_rvx = nullptr, *d_rvz = nullptr, *d_rp = nullptr;
-  // snapshots
-  int wsave = 0, savew = 0, snap_sample = 1;
-  fdmod_snapshot_fn snap_fn = nullptr;
-  void *snap_user = nullptr;
-  float *d_p = nullptr;
-  std::vector<float> h_snap;
-  // energy
-  int energy = 0, eblocks = 0;
-  float *d_E = nullptr, *d_Ek = nullptr, *d_Ep = nullptr, *d_partial = nullptr;
-  std::string err;
-};
-
-static int fail(fdmod_ctx *c, int code, const char *fmt, ...)
+int fail(fdmod_ctx *c, int code, const char *fmt, ...)
 {
   char buf[512];
   va_list ap;
@@ -56,13 +26,8 @@ static int fail(fdmod_ctx *c, int code, const char *fmt, ...)
   return code;
 }
 
-#define HIPCALL(c, x)                                                                              \
-  do {                                                                                             \
-    hipError_t e_ = (x);                                                                           \
-    if (e_ != hipSuccess) return fail((c), FDMOD_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
-  } while (0)
 
-static void dfree(void *p)
+void dfree(void *p)
 {
   if (p) (void)hipFree(p);
 }
@@ -109,6 +74,8 @@ static void release(fdmod_ctx *c)
   dfree(c->d_isx); dfree(c->d_isz); dfree(c->d_src); dfree(c->d_taps);
   dfree(c->d_gx); dfree(c->d_gz); dfree(c->d_rvx); dfree(c->d_rvz); dfree(c->d_rp);
   dfree(c->d_p); dfree(c->d_E); dfree(c->d_Ek); dfree(c->d_Ep); dfree(c->d_partial);
+  dfree(c->d_mu); dfree(c->d_G); dfree(c->d_ckpt); dfree(c->d_stack); dfree(c->d_res); dfree(c->d_red);
+  dfree(c->d_srcadj); dfree(c->d_grad);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -450,7 +417,7 @@ extern "C" int fdmod_set_energy(fdmod_ctx *c, int on)
   return FDMOD_OK;
 }
 
-static SweepArgs sweep_args(fdmod_ctx *c)
+SweepArgs sweep_args(fdmod_ctx *c)
 {
   SweepArgs a;
   float *const *d = c->d.a;
@@ -502,7 +469,7 @@ static int snapshot(fdmod_ctx *c, int it)
 
 // One iteration of the reference loop (BT:651-954): V sweep, velocity sources,
 // S sweep (+ free surface), stress source, snapshot, receivers, energy.
-static int step(fdmod_ctx *c, int it, const SweepArgs &sa)
+int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
 {
   const fdmod_params &p = c->p;
   launch_vsweep(c->stream, p.form, sa);
@@ -529,6 +496,7 @@ static int step(fdmod_ctx *c, int it, const SweepArgs &sa)
     }
     launch_source(c->stream, src);
   }
+  if (!record) return FDMOD_OK;
   if (c->wsave > 0 && it % c->snap_sample == 0) {
     int rc = snapshot(c, it);
     if (rc) return rc;

@@ -1,0 +1,67 @@
+// The modelling context behind the C-ABI handle, shared by the translation units
+// of the host library (fdmod_api.hip: forward loop and batches; fdmod_grad.hip:
+// adjoint and gradient).
+#pragma once
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <string>
+#include <vector>
+
+#include "fdmod_internal.h"
+
+struct fdmod_ctx {
+  fdmod_params p;
+  fdm::Grid g;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  fdm::DevArrays d;
+  float *pool = nullptr;
+  uint8_t *flags = nullptr;
+  bool model_set = false;
+  // source
+  int stype = 0, ns = 0, npass = 0, ntap = 0, seq = 1;
+  int *d_isx = nullptr, *d_isz = nullptr;
+  float *d_src = nullptr;
+  fdm::SrcTap *d_taps = nullptr;
+  // receivers
+  int seistype = 0, mt = 1, nr = 0, ntsample = 0;
+  int *d_gx = nullptr, *d_gz = nullptr;
+  float *d_rvx = nullptr, *d_rvz = nullptr, *d_rp = nullptr;
+  // snapshots
+  int wsave = 0, savew = 0, snap_sample = 1;
+  fdmod_snapshot_fn snap_fn = nullptr;
+  void *snap_user = nullptr;
+  float *d_p = nullptr;
+  std::vector<float> h_snap;
+  // energy
+  int energy = 0, eblocks = 0;
+  float *d_E = nullptr, *d_Ek = nullptr, *d_Ep = nullptr, *d_partial = nullptr;
+  // adjoint / gradient workspace (allocated on first use, kept until destroy)
+  float *d_mu = nullptr;       // five adjoint fields, each Nx*ldz
+  double *d_G = nullptr;       // six imaging sums, each Nx*ldz
+  float *d_ckpt = nullptr;     // checkpointed states
+  size_t n_ckpt = 0;           // floats allocated in d_ckpt
+  float *d_stack = nullptr;    // v_{k+1} of the steps of one checkpoint segment
+  size_t n_stack = 0;
+  float *d_res = nullptr;      // data-space adjoint sources [3][ntsample][nr]
+  double *d_red = nullptr;     // reduction partials (kRedBlocks) and scalars
+  double *d_srcadj = nullptr;  // adjoint of the source time functions [ns][nt]
+  double *d_grad = nullptr;    // gradient [4][nx][nz]
+  std::string err;
+};
+
+// Record a printf-style message as the context's last error and return code.
+int fail(fdmod_ctx *c, int code, const char *fmt, ...);
+
+#define HIPCALL(c, x)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) return fail((c), FDMOD_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+void dfree(void *p);  // hipFree that ignores null
+
+fdm::SweepArgs sweep_args(fdmod_ctx *c);
+// One iteration of the loop; record = false skips snapshots, receivers and energy.
+int step(fdmod_ctx *c, int it, const fdm::SweepArgs &sa, bool record = true);

@@ -87,6 +87,19 @@ struct RecvArgs {
   int mt;                   // receiver sampling (batched launches derive isample from it)
 };
 
+// Arguments of the adjoint sweeps (Bond form).  g is the grid of the launch: for
+// the velocity-sweep adjoint under a free surface its izStart is lowered by kN so the
+// image rows receive their adjoint; izU is the first row of the update range.
+struct AdjArgs {
+  Grid g;
+  int izU;
+  float *mvx, *mvz, *mxx, *mzz, *mxz;  // adjoint state
+  const float *coef1, *coef2, *invrho;
+  const float *c11, *c13, *c15, *c33, *c35, *c55;
+  const float *fvx, *fvz;              // forward velocity v_{k+1} (imaging only)
+  double *G[6];                        // imaging sums for c11, c13, c15, c33, c35, c55
+};
+
 // ---- launchers (kernels_setup.hip) ----
 void launch_pad(hipStream_t s, const float *raw, float *dst, int nz, int nx, const Grid &g);
 void launch_meal(hipStream_t s, const Grid &g, float *vp, float *vs, float *rho, float *D, float *Beta,
@@ -116,5 +129,20 @@ void launch_energy_partial(hipStream_t s, const Grid &g, int nx, int nz, const D
                            int nblocks);
 void launch_energy_final(hipStream_t s, const float *partial, int nblocks, float dx, float dz, float *E, float *Ek,
                          float *Ep, int it);
+
+// ---- launchers (kernels_adjoint.hip) ----
+void launch_adj_ssweep(hipStream_t s, const AdjArgs &a);
+void launch_adj_vsweep(hipStream_t s, const AdjArgs &a);
+void launch_image(hipStream_t s, const AdjArgs &a);
+void launch_adj_fs(hipStream_t s, const AdjArgs &a);
+void launch_adj_inject(hipStream_t s, int nr, int ldz, const int *gx, const int *gz, const float *r1,
+                       const float *r2, float *f1, float *f2, float scale);
+void launch_adj_source(hipStream_t s, const SourceArgs &a, const float *ma, const float *mb, double *out);
+void launch_residual(hipStream_t s, const float *d, float *r, size_t n, double *partial, int nb, double *out);
+void launch_dot(hipStream_t s, const float *a, const float *bf, const double *bd, size_t n, double *partial, int nb,
+                double *out);
+void launch_chain_bond(hipStream_t s, const Grid &g, int nx, int nz, int fs_row, const DevArrays &d,
+                       const double *G, size_t gstride, double theta_scale, double *grad);
+void launch_rescale_frozen(hipStream_t s, const Grid &g, float *vp, float *vs, float *rho, const float *Beta);
 
 }  // namespace fdm

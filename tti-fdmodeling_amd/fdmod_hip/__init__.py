@@ -38,6 +38,7 @@ class Modeling:
         self.Nz, self.Nx, self.e, self.izStart, self.izEnd, self.ixStart, self.ixEnd = list(d)
         self.nr = 0
         self.ntsample = 0
+        self.ns = 0
         self._snap_cb = None
 
     def close(self):
@@ -68,6 +69,7 @@ class Modeling:
         src = _f32(src).reshape(len(posx), self.nt)
         self._check(self.L.fdmod_set_source(self.c, stype, len(posx), _ptr(posx, ct.c_int), _ptr(posz, ct.c_int),
                                             _ptr(src, ct.c_float)))
+        self.ns = len(posx)
 
     def set_receivers(self, seistype, mt, posx, posz):
         posx = np.ascontiguousarray(posx, dtype=np.int32)
@@ -126,6 +128,50 @@ class Modeling:
         if a.size != self.Nx * self.Nz:
             raise ValueError("padded arrays must have Nx*Nz samples")
         self._check(self.L.fdmod_set_array(self.c, AID[name], _ptr(a, ct.c_float)))
+
+    def update_model(self, vp, vs, rho, eps, delta, theta):
+        """Replace the model keeping the absorbing layers of the last set_model."""
+        arrs = [_f32(a).reshape(-1) for a in (vp, vs, rho, eps, delta, theta)]
+        for a in arrs:
+            if a.size != self.nz * self.nx:
+                raise ValueError("model arrays must have nz*nx samples")
+        self._check(self.L.fdmod_update_model(self.c, *[_ptr(a, ct.c_float) for a in arrs]))
+
+    def _series(self, arrs):
+        out = []
+        for a in arrs:
+            if a is None:
+                out.append(None)
+                continue
+            a = _f32(a).reshape(-1)
+            if a.size != self.nr * self.ntsample:
+                raise ValueError("data series must have ntrace*ntsample samples")
+            out.append(a)
+        return out, [None if a is None else _ptr(a, ct.c_float) for a in out]
+
+    def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, ncheckpoints=0):
+        """J = 1/2 |d - obs|^2 over the given components and dJ/d(vp, eps, delta, theta[deg])
+        as a float64 array (4, nx, nz); runs the forward from rest (Bond form)."""
+        keep, ptrs = self._series((obs_vx, obs_vz, obs_p))
+        J = ct.c_double(0)
+        g = np.empty(4 * self.nx * self.nz, dtype=np.float64)
+        self._check(self.L.fdmod_gradient(self.c, *ptrs, int(ncheckpoints), ct.byref(J), _ptr(g, ct.c_double)))
+        return J.value, g.reshape(4, self.nx, self.nz)
+
+    def adjoint_source(self, y_vx=None, y_vz=None, y_p=None):
+        """F^T y: the adjoint of the source time functions, float64 (ns, nt)."""
+        keep, ptrs = self._series((y_vx, y_vz, y_p))
+        ns = self.ns
+        out = np.empty(ns * self.nt, dtype=np.float64)
+        self._check(self.L.fdmod_adjoint_source(self.c, *ptrs, _ptr(out, ct.c_double)))
+        return out.reshape(ns, self.nt)
+
+    def adjoint_dot(self, y_vx=None, y_vz=None, y_p=None):
+        """(<F s, y>, <s, F^T y>) reduced in FP64 on the device."""
+        keep, ptrs = self._series((y_vx, y_vz, y_p))
+        lhs, rhs = ct.c_double(0), ct.c_double(0)
+        self._check(self.L.fdmod_adjoint_dot(self.c, *ptrs, ct.byref(lhs), ct.byref(rhs)))
+        return lhs.value, rhs.value
 
     def time_sweeps(self, reps=20):
         """Mean device ms of one velocity-sweep and one stress-sweep launch."""

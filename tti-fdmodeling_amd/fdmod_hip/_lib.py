@@ -66,6 +66,10 @@ SIGNATURES = {
     "fdmod_batch_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _dp]),
     "fdmod_batch_reset": (ct.c_int, [ct.c_void_p]),
     "fdmod_batch_get_traces": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _fp]),
+    "fdmod_gradient": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp, ct.c_int, _dp, _dp]),
+    "fdmod_update_model": (ct.c_int, [ct.c_void_p] + [_fp] * 6),
+    "fdmod_adjoint_source": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp, _dp]),
+    "fdmod_adjoint_dot": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp, _dp, _dp]),
 }
 
 _LIB = None
