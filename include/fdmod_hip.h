@@ -182,6 +182,32 @@ int fdmod_batch_get_traces(fdmod_batch *b, int shot, int component, float *out);
  * rest; the wavefield is not.  vs and rho are not differentiated. */
 int fdmod_gradient(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, const float *obs_p, int ncheckpoints,
                    double *misfit, double *grad);
+
+/* Storage of the wavefields the gradient keeps.  FP16 stores each field as FP16
+ * with one power-of-two scale per column and 64-row block (stencil arithmetic
+ * stays FP32).  history = 1 keeps v_{k+1} of every step (2*nt fields, no
+ * recomputation); history = 0 keeps ncheckpoints states and recomputes each
+ * segment.  fdmod_gradient is {ncheckpoints, FDMOD_STORE_F32, 0}. */
+#define FDMOD_STORE_F32 0
+#define FDMOD_STORE_F16 1
+typedef struct fdmod_grad_opts {
+  int ncheckpoints; /* <= 0: ceil(sqrt(nt))                    */
+  int storage;      /* FDMOD_STORE_F32 or FDMOD_STORE_F16       */
+  int history;      /* 0: checkpoints + recomputation; 1: full history */
+} fdmod_grad_opts;
+int fdmod_gradient_opts(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, const float *obs_p,
+                        const fdmod_grad_opts *opts, double *misfit, double *grad);
+
+/* Misfit gradient of every shot of the batch (Bond form), each as fdmod_gradient_opts
+ * would compute it, with the shots of a group advancing together on their own streams
+ * (groups are as large as free device memory allows for the chosen storage).  obs_* are
+ * arrays of nshots pointers (the array itself NULL = component absent).  With x0, shot s
+ * covers model columns [x0[s], x0[s]+nx[s]) of an nx_full-column model and grad is the
+ * FP64 sum over shots, [4][nx_full][nz]; with x0 NULL, grad holds the per-shot gradients
+ * one after another ([shot][4][nx[s]][nz]).  *misfit is the sum of the shots' misfits. */
+int fdmod_batch_gradient(fdmod_batch *b, const float *const *obs_vx, const float *const *obs_vz,
+                         const float *const *obs_p, const fdmod_grad_opts *opts, const int *x0, int nx_full,
+                         double *misfit, double *grad);
 /* Replace the model keeping the absorbing layers of the last fdmod_set_model (the
  * model the gradient is defined with); same array conventions as fdmod_set_model. */
 int fdmod_update_model(fdmod_ctx *ctx, const float *vp, const float *vs, const float *rho, const float *eps,

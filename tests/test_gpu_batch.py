@@ -78,3 +78,25 @@ def test_autotuner_and_tile_shapes_keep_results():
         b.run()
         for i in range(len(shots)):
             assert np.array_equal(b.traces(i, 4), ref[i]), (lv, ls, i)
+
+
+@pytest.mark.parametrize("storage,history", [("f32", True), ("f16", False)])
+def test_batch_gradient_equals_single_shot_gradients(storage, history):
+    shots = _shots()
+    b = _batch("bond", shots)
+    ns = (NT - 1) // 2 + 1
+    rng = np.random.default_rng(11)
+    obs_p = [rng.standard_normal((len(sh["rx"]), ns)).astype(np.float32) * 1e-6 for sh in shots]
+    J, per = b.gradient(obs_p=obs_p, storage=storage, history=history)
+    x0 = [10, 40, 90, 100]
+    J2, full = b.gradient(obs_p=obs_p, storage=storage, history=history, x0=x0, nx_full=200)
+    ref = np.zeros_like(full)
+    Jsum = 0.0
+    for s, sh in enumerate(shots):
+        g = _single(Modeling, "bond", sh)
+        Js, gs = g.gradient(obs_p=obs_p[s], storage=storage, history=history)
+        assert np.array_equal(per[s], gs), s
+        ref[:, x0[s]:x0[s] + sh["w"]] += gs
+        Jsum += Js
+    assert J == Jsum and J2 == Jsum
+    assert np.array_equal(full, ref)

@@ -74,7 +74,8 @@ static void release(fdmod_ctx *c)
   dfree(c->d_isx); dfree(c->d_isz); dfree(c->d_src); dfree(c->d_taps);
   dfree(c->d_gx); dfree(c->d_gz); dfree(c->d_rvx); dfree(c->d_rvz); dfree(c->d_rp);
   dfree(c->d_p); dfree(c->d_E); dfree(c->d_Ek); dfree(c->d_Ep); dfree(c->d_partial);
-  dfree(c->d_mu); dfree(c->d_G); dfree(c->d_ckpt); dfree(c->d_stack); dfree(c->d_res); dfree(c->d_red);
+  dfree(c->d_mu); dfree(c->d_G); dfree(c->d_snap); dfree(c->d_res); dfree(c->d_red);
+  dfree(c->ckpt.data); dfree(c->ckpt.scale); dfree(c->hist.data); dfree(c->hist.scale);
   dfree(c->d_srcadj); dfree(c->d_grad);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -108,7 +109,13 @@ extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
   g.ixEnd = g.Nx - kN;
   g.izStart = p->freesurface == 0 ? kN : g.e;
   g.izEnd = g.Nz - kN;
+  g.tilesZ = (g.izEnd - g.izStart + TZ - 1) / TZ;
+  // Longest velocity-sweep segment (44, 33, 22 or 11 columns) that still gives about
+  // kMinTiles one-wave blocks: long segments cut halo re-reads, but a small grid
+  // needs short ones to keep enough waves in flight.
   g.segLV = kDefaultSegLV;
+  while (g.segLV > kFlagW && g.tilesZ * ((g.ixEnd - g.ixStart + g.segLV - 1) / g.segLV) < kMinTiles)
+    g.segLV -= kFlagW;
   g.segLS = kDefaultSegLS;
   if (const char *sl = getenv("FDMOD_SEGLV")) {  // tile-shape overrides (tuning experiments)
     int v = atoi(sl);
@@ -118,7 +125,6 @@ extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
     int v = atoi(sl);
     if (v >= 1 && v <= 4096) g.segLS = v;
   }
-  g.tilesZ = (g.izEnd - g.izStart + TZ - 1) / TZ;
   g.flagsX = (g.ixEnd - g.ixStart + kFlagW - 1) / kFlagW;
   g.invdx = (float)(1.0 / p->dx);
   g.invdz = (float)(1.0 / p->dz);
@@ -623,27 +629,8 @@ extern "C" int fdmod_time_sweeps(fdmod_ctx *c, int reps, double *ms_v, double *m
 // on the time index read the loop counter from device memory, so one captured
 // chunk serves the whole run.
 
-struct fdmod_batch {
-  fdmod_params p;
-  std::vector<fdmod_ctx *> shots;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int segLV = kDefaultSegLV, segLS = kDefaultSegLS;
-  bool prepared = false;
-  SweepArgs *d_sweep = nullptr;
-  int *d_startV = nullptr, *d_startS = nullptr, totalV = 0, totalS = 0;
-  SourceArgs *d_srcV = nullptr, *d_srcS = nullptr;
-  int srcBlocksV = 0, srcBlocksS = 0;
-  RecvArgs *d_rec = nullptr;
-  int maxnr = 0;
-  int *d_step = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  int chunk = 32;
-  std::string err;
-};
 
-static int bfail(fdmod_batch *b, int code, const char *fmt, ...)
+int bfail(fdmod_batch *b, int code, const char *fmt, ...)
 {
   char buf[512];
   va_list ap;
@@ -654,11 +641,6 @@ static int bfail(fdmod_batch *b, int code, const char *fmt, ...)
   return code;
 }
 
-#define BHIP(b, x)                                                                                  \
-  do {                                                                                              \
-    hipError_t e_ = (x);                                                                            \
-    if (e_ != hipSuccess) return bfail((b), FDMOD_EDEVICE, "%s: %s", #x, hipGetErrorString(e_));   \
-  } while (0)
 
 static void batch_unprepare(fdmod_batch *b)
 {

@@ -10,6 +10,14 @@
 
 #include "fdmod_internal.h"
 
+// Device slots of padded fields, stored as FP32 or as FP16 with per-(column, 64-row
+// block) inverse scales.
+struct FieldStore {
+  void *data = nullptr;
+  float *scale = nullptr;
+  size_t cap_data = 0, cap_scale = 0;  // bytes allocated
+};
+
 struct fdmod_ctx {
   fdmod_params p;
   fdm::Grid g;
@@ -40,10 +48,9 @@ struct fdmod_ctx {
   // adjoint / gradient workspace (allocated on first use, kept until destroy)
   float *d_mu = nullptr;       // five adjoint fields, each Nx*ldz
   double *d_G = nullptr;       // six imaging sums, each Nx*ldz
-  float *d_ckpt = nullptr;     // checkpointed states
-  size_t n_ckpt = 0;           // floats allocated in d_ckpt
-  float *d_stack = nullptr;    // v_{k+1} of the steps of one checkpoint segment
-  size_t n_stack = 0;
+  float *d_snap = nullptr;     // mu_sigma snapshots of one imaging block (kImgK x 3 fields)
+  FieldStore ckpt;             // checkpointed states (5 fields per slot)
+  FieldStore hist;             // v_{k+1} per step (2 fields per slot): one segment or the whole run
   float *d_res = nullptr;      // data-space adjoint sources [3][ntsample][nr]
   double *d_red = nullptr;     // reduction partials (kRedBlocks) and scalars
   double *d_srcadj = nullptr;  // adjoint of the source time functions [ns][nt]
@@ -61,6 +68,37 @@ int fail(fdmod_ctx *c, int code, const char *fmt, ...);
   } while (0)
 
 void dfree(void *p);  // hipFree that ignores null
+
+// A batch of shots (fdmod_api.hip: batched forward loop; fdmod_grad.hip: batched
+// gradients).  Each shot is a full modelling context with its own stream.
+struct fdmod_batch {
+  fdmod_params p;
+  std::vector<fdmod_ctx *> shots;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int segLV = fdm::kDefaultSegLV, segLS = fdm::kDefaultSegLS;
+  bool prepared = false;
+  fdm::SweepArgs *d_sweep = nullptr;
+  int *d_startV = nullptr, *d_startS = nullptr, totalV = 0, totalS = 0;
+  fdm::SourceArgs *d_srcV = nullptr, *d_srcS = nullptr;
+  int srcBlocksV = 0, srcBlocksS = 0;
+  fdm::RecvArgs *d_rec = nullptr;
+  int maxnr = 0;
+  int *d_step = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int chunk = 32;
+  std::string err;
+};
+
+int bfail(fdmod_batch *b, int code, const char *fmt, ...);
+
+#define BHIP(b, x)                                                                                  \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) return bfail((b), FDMOD_EDEVICE, "%s: %s", #x, hipGetErrorString(e_));   \
+  } while (0)
+
 
 fdm::SweepArgs sweep_args(fdmod_ctx *c);
 // One iteration of the loop; record = false skips snapshots, receivers and energy.

@@ -2,15 +2,16 @@
 // modelling operator, the device dot-product test and the frozen-absorber model
 // update (Bond form).  Declarations are in include/fdmod_hip.h.
 //
-// Gradient schedule (checkpointed recomputation):
-//   1. forward from rest, recording traces; every seg-th state (5 fields) is copied
-//      into the checkpoint store;
+// Gradient schedule:
+//   1. forward from rest, recording traces; with checkpoints every seg-th state
+//      (5 fields) is stored, with the full history v_{k+1} of every step;
 //   2. residuals r = d - obs and J = sum r^2 / 2, reduced in FP64 on the device;
-//   3. for each checkpoint segment, last first: restore its state, recompute its
-//      steps storing v_{k+1} of each, then run the adjoint steps of the segment
-//      backwards, imaging with the stored velocities;
+//   3. for each checkpoint segment, last first: restore its state and recompute its
+//      steps storing v_{k+1} of each (skipped with the full history), then run the
+//      adjoint steps of the segment backwards, imaging with the stored velocities;
 //   4. chain rule from the six stiffness sums to (vp, eps, delta, theta) and the
 //      transpose of the edge padding.
+// Stored fields are FP32 copies or FP16 with per-(column, 64-row block) scales.
 #include <math.h>
 #include <string.h>
 
@@ -44,24 +45,81 @@ int workspace(fdmod_ctx *c, bool need_G)
   if (!c->d_mu) HIPCALL(c, hipMalloc(&c->d_mu, kNState * n * sizeof(float)));
   if (!c->d_red) HIPCALL(c, hipMalloc(&c->d_red, (kRedBlocks + 8) * sizeof(double)));
   if (need_G && !c->d_G) HIPCALL(c, hipMalloc(&c->d_G, 6 * n * sizeof(double)));
+  if (need_G && !c->d_snap) HIPCALL(c, hipMalloc(&c->d_snap, (size_t)kImgK * 3 * n * sizeof(float)));
   dfree(c->d_res);
   c->d_res = nullptr;
   HIPCALL(c, hipMalloc(&c->d_res, 3 * (size_t)c->nr * c->ntsample * sizeof(float)));
   return FDMOD_OK;
 }
 
-int grow(fdmod_ctx *c, float **p, size_t &have, size_t need)
+int reserve(fdmod_ctx *c, void **p, size_t &have, size_t need)
 {
   if (have >= need) return FDMOD_OK;
   dfree(*p);
   *p = nullptr;
   have = 0;
-  if (hipMalloc(p, need * sizeof(float)) != hipSuccess) {
+  if (hipMalloc(p, need) != hipSuccess) {
     (void)hipGetLastError();
-    return fail(c, FDMOD_ENOMEM, "cannot allocate %.2f GB of adjoint workspace", need * 4.0 / 1e9);
+    return fail(c, FDMOD_ENOMEM, "cannot allocate %.2f GB of gradient storage", need / 1e9);
   }
   have = need;
   return FDMOD_OK;
+}
+
+size_t scales_per_field(const fdmod_ctx *c) { return (size_t)c->g.Nx * (c->g.ldz / TZ); }
+
+// Capacity for nslots slots of nf fields.
+int store_reserve(fdmod_ctx *c, FieldStore &st, size_t nslots, int nf, bool f16)
+{
+  const size_t n = field_floats(c) * nslots * nf;
+  int rc = reserve(c, &st.data, st.cap_data, n * (f16 ? sizeof(_Float16) : sizeof(float)));
+  if (rc || !f16) return rc;
+  return reserve(c, (void **)&st.scale, st.cap_scale, scales_per_field(c) * nslots * nf * sizeof(float));
+}
+
+// Copy (FP32) or pack (FP16) the nf fields into / out of a slot.
+int store_io(fdmod_ctx *c, FieldStore &st, size_t slot, int nf, float *const *fields, bool f16, bool save)
+{
+  const size_t n = field_floats(c);
+  if (!f16) {
+    float *base = (float *)st.data + slot * nf * n;
+    for (int f = 0; f < nf; f++)
+      HIPCALL(c, hipMemcpyAsync(save ? base + f * n : fields[f], save ? fields[f] : base + f * n, n * sizeof(float),
+                                hipMemcpyDeviceToDevice, c->stream));
+    return FDMOD_OK;
+  }
+  Pack16 pk{};
+  pk.nx = c->g.Nx;
+  pk.ldz = c->g.ldz;
+  pk.nf = nf;
+  const size_t ns = scales_per_field(c);
+  for (int f = 0; f < nf; f++) {
+    pk.src[f] = fields[f];
+    pk.dst[f] = (_Float16 *)st.data + (slot * nf + f) * n;
+    pk.inv[f] = st.scale + (slot * nf + f) * ns;
+  }
+  launch_pack16(c->stream, pk, save);
+  return FDMOD_OK;
+}
+
+// Imaging block entry j: snapshot slot j and history slot `slot` (fields vx, vz).
+void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
+{
+  const size_t n = field_floats(c);
+  for (int q = 0; q < 3; q++) b.lam[j][q] = c->d_snap + ((size_t)j * 3 + q) * n;
+  const FieldStore &st = c->hist;
+  if (!f16) {
+    b.fv[j][0] = (const float *)st.data + slot * 2 * n;
+    b.fv[j][1] = b.fv[j][0] + n;
+    b.hv[j][0] = b.hv[j][1] = nullptr;
+    return;
+  }
+  const size_t ns = scales_per_field(c);
+  b.hv[j][0] = (const _Float16 *)st.data + slot * 2 * n;
+  b.hv[j][1] = b.hv[j][0] + n;
+  b.sv[j][0] = st.scale + slot * 2 * ns;
+  b.sv[j][1] = b.sv[j][0] + ns;
+  b.fv[j][0] = b.fv[j][1] = nullptr;
 }
 
 AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
@@ -79,6 +137,7 @@ AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
   a.coef1 = d[FDMOD_A_COEF1]; a.coef2 = d[FDMOD_A_COEF2]; a.invrho = d[FDMOD_A_INVRHO];
   a.c11 = d[FDMOD_A_C11]; a.c13 = d[FDMOD_A_C13]; a.c15 = d[FDMOD_A_C15];
   a.c33 = d[FDMOD_A_C33]; a.c35 = d[FDMOD_A_C35]; a.c55 = d[FDMOD_A_C55];
+  a.snap[0] = a.snap[1] = a.snap[2] = nullptr;
   for (int j = 0; j < 6; j++) a.G[j] = c->d_G ? c->d_G + j * n : nullptr;
   return a;
 }
@@ -129,7 +188,8 @@ int upload_data(fdmod_ctx *c, const float *hvx, const float *hvz, const float *h
 }
 
 // One adjoint step: the transpose of loop iteration k.
-void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, const float *fvx, const float *fvz, double *srcadj)
+// snap_slot >= 0: the V-adjoint copies the imaged mu_sigma into that snapshot slot.
+void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *srcadj)
 {
   const fdmod_params &p = c->p;
   hipStream_t s = c->stream;
@@ -140,11 +200,6 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, const float *fvx, const fl
   if (rec && ds.p) launch_adj_inject(s, c->nr, c->g.ldz, c->d_gx, c->d_gz, ds.p + so, ds.p + so, a.mxx, a.mzz, 0.5f);
   if (srcadj && c->ns > 0 && c->stype == 1) launch_adj_source(s, source_args(c, it), a.mxx, a.mzz, srcadj);
   if (p.freesurface) launch_adj_fs(s, a);
-  if (fvx) {
-    a.fvx = fvx;
-    a.fvz = fvz;
-    launch_image(s, a);
-  }
   launch_adj_ssweep(s, a);
   if (rec && ds.vx)
     launch_adj_inject(s, c->nr, c->g.ldz, c->d_gx, c->d_gz, ds.vx + so, ds.vz + so, a.mvx, a.mvz, 1.0f);
@@ -153,7 +208,10 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, const float *fvx, const fl
     const float *mb = (c->stype == 5 || c->stype == 6) ? nullptr : a.mvz;
     launch_adj_source(s, source_args(c, it), ma, mb, srcadj);
   }
-  launch_adj_vsweep(s, adj_args(c, true));
+  AdjArgs av = adj_args(c, true);
+  if (snap_slot >= 0)
+    for (int q = 0; q < 3; q++) av.snap[q] = c->d_snap + ((size_t)snap_slot * 3 + q) * field_floats(c);
+  launch_adj_vsweep(s, av);
 }
 
 int zero_adjoint(fdmod_ctx *c)
@@ -164,16 +222,17 @@ int zero_adjoint(fdmod_ctx *c)
   return FDMOD_OK;
 }
 
-int copy_state(fdmod_ctx *c, float *store, bool save)
+int state_io(fdmod_ctx *c, size_t slot, bool f16, bool save)
 {
-  const size_t n = field_floats(c);
-  for (int f = 0; f < kNState; f++) {
-    float *field = c->d.a[kStateIds[f]];
-    float *slot = store + f * n;
-    HIPCALL(c, hipMemcpyAsync(save ? slot : field, save ? field : slot, n * sizeof(float), hipMemcpyDeviceToDevice,
-                              c->stream));
-  }
-  return FDMOD_OK;
+  float *f[kNState];
+  for (int i = 0; i < kNState; i++) f[i] = c->d.a[kStateIds[i]];
+  return store_io(c, c->ckpt, slot, kNState, f, f16, save);
+}
+
+int velocity_io(fdmod_ctx *c, size_t slot, bool f16)
+{
+  float *f[2] = {c->d.a[FDMOD_A_VX], c->d.a[FDMOD_A_VZ]};
+  return store_io(c, c->hist, slot, 2, f, f16, true);
 }
 
 // Run the whole adjoint loop with the given data-space sources (no imaging).
@@ -181,7 +240,7 @@ int adjoint_only(fdmod_ctx *c, const DataSrc &ds, double *srcadj)
 {
   int rc = zero_adjoint(c);
   if (rc) return rc;
-  for (int k = c->p.nt - 1; k >= 0; k--) adj_step(c, k, ds, nullptr, nullptr, srcadj);
+  for (int k = c->p.nt - 1; k >= 0; k--) adj_step(c, k, ds, -1, srcadj);
   HIPCALL(c, hipStreamSynchronize(c->stream));
   HIPCALL(c, hipGetLastError());
   return FDMOD_OK;
@@ -194,69 +253,190 @@ int reduce_to_host(fdmod_ctx *c, double *out)
   return FDMOD_OK;
 }
 
+
+// One gradient computation as resumable phases, so that a batch can interleave the
+// launches of several shots (each on its own stream).
+struct GradRun {
+  fdmod_ctx *c = nullptr;
+  bool f16 = false, history = false;
+  int ncp = 1, seg = 1;
+  DataSrc ds;
+  SweepArgs sa;
+  int sgi = 0, k = 0, phase = 0;  // backward cursor: segment, step, 0 = recompute / 1 = adjoint
+  ImgBlock blk{};
+  bool done = false;
+};
+
+int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_vz, const float *obs_p,
+               const fdmod_grad_opts *opts)
+{
+  if (!opts) return fail(c, FDMOD_EINVAL, "fdmod_gradient: opts must not be NULL");
+  if (opts->storage != FDMOD_STORE_F32 && opts->storage != FDMOD_STORE_F16)
+    return fail(c, FDMOD_EINVAL, "fdmod_gradient: storage must be FDMOD_STORE_F32 or FDMOD_STORE_F16");
+  int rc = check_ready(c, "fdmod_gradient");
+  if (rc) return rc;
+  const fdmod_params &p = c->p;
+  r = GradRun{};
+  r.c = c;
+  r.f16 = opts->storage == FDMOD_STORE_F16;
+  r.history = opts->history != 0;
+  if ((rc = workspace(c, true))) return rc;
+  if ((rc = upload_data(c, obs_vx, obs_vz, obs_p, r.ds))) return rc;
+  // storage: the whole velocity history, or ncp states one every seg steps plus the
+  // velocity history of one segment
+  int ncp = opts->ncheckpoints > 0 ? opts->ncheckpoints : (int)ceil(sqrt((double)p.nt));
+  ncp = std::min(std::max(ncp, 1), p.nt);
+  r.seg = (p.nt + ncp - 1) / ncp;
+  r.ncp = (p.nt + r.seg - 1) / r.seg;
+  if (r.history) {
+    r.seg = p.nt;
+    r.ncp = 1;
+    if ((rc = store_reserve(c, c->hist, p.nt, 2, r.f16))) return rc;
+  } else {
+    if ((rc = store_reserve(c, c->ckpt, r.ncp, kNState, r.f16))) return rc;
+    if ((rc = store_reserve(c, c->hist, r.seg, 2, r.f16))) return rc;
+  }
+  if ((rc = fdmod_reset(c))) return rc;
+  r.sa = sweep_args(c);
+  return zero_adjoint(c);
+}
+
+// Forward step k from rest: checkpoint or history store, then the loop iteration.
+int grad_forward(GradRun &r, int k)
+{
+  fdmod_ctx *c = r.c;
+  const fdmod_params &p = c->p;
+  int rc;
+  if (!r.history && k % r.seg == 0 && (rc = state_io(c, k / r.seg, r.f16, true))) return rc;
+  if ((rc = step(c, p.mode == 1 ? k : p.nt - 1 - k, r.sa, true))) return rc;
+  if (r.history && (rc = velocity_io(c, k, r.f16))) return rc;
+  return FDMOD_OK;
+}
+
+// Residuals (in place over the uploaded observations); J accumulates on the device.
+int grad_residual(GradRun &r)
+{
+  fdmod_ctx *c = r.c;
+  const size_t nrec = (size_t)c->nr * c->ntsample;
+  HIPCALL(c, hipMemsetAsync(c->d_red + kRedBlocks, 0, sizeof(double), c->stream));
+  if (r.ds.vx) {
+    launch_residual(c->stream, c->d_rvx, c->d_res, nrec, c->d_red, kRedBlocks, c->d_red + kRedBlocks);
+    launch_residual(c->stream, c->d_rvz, c->d_res + nrec, nrec, c->d_red, kRedBlocks, c->d_red + kRedBlocks);
+  }
+  if (r.ds.p)
+    launch_residual(c->stream, c->d_rp, c->d_res + 2 * nrec, nrec, c->d_red, kRedBlocks, c->d_red + kRedBlocks);
+  r.sgi = r.ncp - 1;
+  r.phase = r.history ? 1 : 0;
+  r.k = r.history ? r.c->p.nt - 1 : r.sgi * r.seg;
+  return FDMOD_OK;
+}
+
+// One unit of the backward pass: a recomputed forward step (checkpoint mode) or an
+// adjoint step.  Segments run last first; the imaging runs once per kImgK adjoint
+// steps and at each segment's end, before the next recomputation reuses the slots.
+int grad_backward_next(GradRun &r)
+{
+  fdmod_ctx *c = r.c;
+  const fdmod_params &p = c->p;
+  const int k0 = r.sgi * r.seg, k1 = std::min(p.nt, k0 + r.seg);
+  int rc;
+  if (r.phase == 0) {
+    if (r.k == k0 && (rc = state_io(c, r.sgi, r.f16, false))) return rc;
+    if ((rc = step(c, p.mode == 1 ? r.k : p.nt - 1 - r.k, r.sa, false))) return rc;
+    if ((rc = velocity_io(c, r.k - k0, r.f16))) return rc;
+    if (++r.k == k1) {
+      r.phase = 1;
+      r.k = k1 - 1;
+    }
+    return FDMOD_OK;
+  }
+  adj_step(c, r.k, r.ds, r.blk.nk, nullptr);
+  add_to_block(c, r.blk, r.blk.nk, r.k - k0, r.f16);
+  if (++r.blk.nk == kImgK || r.k == k0) {
+    launch_image_blk(c->stream, adj_args(c, false), r.blk);
+    r.blk = ImgBlock{};
+  }
+  if (r.k > k0) {
+    r.k--;
+  } else if (r.sgi > 0) {
+    r.sgi--;
+    r.phase = 0;
+    r.k = r.sgi * r.seg;
+  } else {
+    r.done = true;
+  }
+  return FDMOD_OK;
+}
+
+// Release the gradient storage and workspace of a context.
+void release_grad(fdmod_ctx *c)
+{
+  dfree(c->d_mu); dfree(c->d_G); dfree(c->d_snap);
+  dfree(c->ckpt.data); dfree(c->ckpt.scale); dfree(c->hist.data); dfree(c->hist.scale);
+  c->d_mu = nullptr; c->d_G = nullptr; c->d_snap = nullptr;
+  c->ckpt = FieldStore{};
+  c->hist = FieldStore{};
+}
+
+// Device bytes one shot's gradient needs beyond its modelling arrays.
+size_t grad_bytes(const fdmod_ctx *c, const fdmod_grad_opts &o)
+{
+  const size_t n = field_floats(c), nt = c->p.nt;
+  const size_t esz = o.storage == FDMOD_STORE_F16 ? 2 : 4;
+  size_t b = n * (kNState * 4 + 6 * 8 + kImgK * 3 * 4);
+  if (o.history) {
+    b += nt * 2 * n * esz;
+  } else {
+    int ncp = o.ncheckpoints > 0 ? o.ncheckpoints : (int)ceil(sqrt((double)nt));
+    ncp = std::min(std::max(ncp, 1), (int)nt);
+    const size_t seg = (nt + ncp - 1) / ncp;
+    b += ((nt + seg - 1) / seg * kNState + seg * 2) * n * esz;
+  }
+  return b + b / 16;  // block scales and slack
+}
+
+// Chain rule and unpadding into c->d_grad ([4][nx][nz], device).
+int grad_finish(GradRun &r)
+{
+  fdmod_ctx *c = r.c;
+  const fdmod_params &p = c->p;
+  if (!c->d_grad) HIPCALL(c, hipMalloc(&c->d_grad, 4 * (size_t)p.nx * p.nz * sizeof(double)));
+  const int fs_row = p.freesurface ? c->g.e : -1;
+  launch_chain_bond(c->stream, c->g, p.nx, p.nz, fs_row, c->d, c->d_G, field_floats(c),
+                    (double)p.scalar / 180.0 * kPI, c->d_grad);
+  HIPCALL(c, hipGetLastError());
+  return FDMOD_OK;
+}
+
 }  // namespace
+
+extern "C" int fdmod_gradient_opts(fdmod_ctx *c, const float *obs_vx, const float *obs_vz, const float *obs_p,
+                                   const fdmod_grad_opts *opts, double *misfit, double *grad)
+{
+  if (!c) return FDMOD_EINVAL;
+  if (!misfit || !grad) return fail(c, FDMOD_EINVAL, "fdmod_gradient: misfit and grad must not be NULL");
+  GradRun r;
+  int rc = grad_begin(r, c, obs_vx, obs_vz, obs_p, opts);
+  if (rc) return rc;
+  for (int k = 0; k < c->p.nt; k++)
+    if ((rc = grad_forward(r, k))) return rc;
+  if ((rc = grad_residual(r))) return rc;
+  while (!r.done)
+    if ((rc = grad_backward_next(r))) return rc;
+  if ((rc = grad_finish(r))) return rc;
+  HIPCALL(c, hipMemcpyAsync(misfit, c->d_red + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipMemcpyAsync(grad, c->d_grad, 4 * (size_t)c->p.nx * c->p.nz * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  HIPCALL(c, hipGetLastError());
+  return FDMOD_OK;
+}
 
 extern "C" int fdmod_gradient(fdmod_ctx *c, const float *obs_vx, const float *obs_vz, const float *obs_p,
                               int ncheckpoints, double *misfit, double *grad)
 {
-  if (!c) return FDMOD_EINVAL;
-  if (!misfit || !grad) return fail(c, FDMOD_EINVAL, "fdmod_gradient: misfit and grad must not be NULL");
-  int rc = check_ready(c, "fdmod_gradient");
-  if (rc) return rc;
-  const fdmod_params &p = c->p;
-  if ((rc = workspace(c, true))) return rc;
-  DataSrc ds;
-  if ((rc = upload_data(c, obs_vx, obs_vz, obs_p, ds))) return rc;
-  // checkpoint layout: ncp states, one every seg steps
-  int ncp = ncheckpoints > 0 ? ncheckpoints : (int)ceil(sqrt((double)p.nt));
-  ncp = std::min(std::max(ncp, 1), p.nt);
-  const int seg = (p.nt + ncp - 1) / ncp;
-  ncp = (p.nt + seg - 1) / seg;
-  const size_t n = field_floats(c);
-  if ((rc = grow(c, &c->d_ckpt, c->n_ckpt, (size_t)ncp * kNState * n))) return rc;
-  if ((rc = grow(c, &c->d_stack, c->n_stack, (size_t)seg * 2 * n))) return rc;
-  // 1. forward with checkpoints
-  if ((rc = fdmod_reset(c))) return rc;
-  SweepArgs sa = sweep_args(c);
-  for (int k = 0; k < p.nt; k++) {
-    if (k % seg == 0 && (rc = copy_state(c, c->d_ckpt + (size_t)(k / seg) * kNState * n, true))) return rc;
-    if ((rc = step(c, p.mode == 1 ? k : p.nt - 1 - k, sa, true))) return rc;
-  }
-  // 2. residuals (in place over the uploaded observations) and the misfit
-  const size_t nrec = (size_t)c->nr * c->ntsample;
-  HIPCALL(c, hipMemsetAsync(c->d_red + kRedBlocks, 0, sizeof(double), c->stream));
-  if (ds.vx) {
-    launch_residual(c->stream, c->d_rvx, c->d_res, nrec, c->d_red, kRedBlocks, c->d_red + kRedBlocks);
-    launch_residual(c->stream, c->d_rvz, c->d_res + nrec, nrec, c->d_red, kRedBlocks, c->d_red + kRedBlocks);
-  }
-  if (ds.p) launch_residual(c->stream, c->d_rp, c->d_res + 2 * nrec, nrec, c->d_red, kRedBlocks, c->d_red + kRedBlocks);
-  if ((rc = reduce_to_host(c, misfit))) return rc;
-  // 3. segments backwards: recompute, then adjoint with imaging
-  if ((rc = zero_adjoint(c))) return rc;
-  for (int sgi = ncp - 1; sgi >= 0; sgi--) {
-    const int k0 = sgi * seg, k1 = std::min(p.nt, k0 + seg);
-    if ((rc = copy_state(c, c->d_ckpt + (size_t)sgi * kNState * n, false))) return rc;
-    for (int k = k0; k < k1; k++) {
-      if ((rc = step(c, p.mode == 1 ? k : p.nt - 1 - k, sa, false))) return rc;
-      float *slot = c->d_stack + (size_t)(k - k0) * 2 * n;
-      HIPCALL(c, hipMemcpyAsync(slot, c->d.a[FDMOD_A_VX], n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-      HIPCALL(c, hipMemcpyAsync(slot + n, c->d.a[FDMOD_A_VZ], n * sizeof(float), hipMemcpyDeviceToDevice,
-                                c->stream));
-    }
-    for (int k = k1 - 1; k >= k0; k--) {
-      const float *slot = c->d_stack + (size_t)(k - k0) * 2 * n;
-      adj_step(c, k, ds, slot, slot + n, nullptr);
-    }
-  }
-  // 4. chain rule and unpadding
-  if (!c->d_grad) HIPCALL(c, hipMalloc(&c->d_grad, 4 * (size_t)p.nx * p.nz * sizeof(double)));
-  const int fs_row = p.freesurface ? c->g.e : -1;
-  launch_chain_bond(c->stream, c->g, p.nx, p.nz, fs_row, c->d, c->d_G, n, (double)p.scalar / 180.0 * kPI, c->d_grad);
-  HIPCALL(c, hipMemcpyAsync(grad, c->d_grad, 4 * (size_t)p.nx * p.nz * sizeof(double), hipMemcpyDeviceToHost,
-                            c->stream));
-  HIPCALL(c, hipStreamSynchronize(c->stream));
-  HIPCALL(c, hipGetLastError());
-  return FDMOD_OK;
+  fdmod_grad_opts o{ncheckpoints, FDMOD_STORE_F32, 0};
+  return fdmod_gradient_opts(c, obs_vx, obs_vz, obs_p, &o, misfit, grad);
 }
 
 extern "C" int fdmod_adjoint_source(fdmod_ctx *c, const float *y_vx, const float *y_vz, const float *y_p,
@@ -347,4 +527,80 @@ extern "C" int fdmod_update_model(fdmod_ctx *c, const float *vp, const float *vs
   if (e != hipSuccess || (e = hipGetLastError()) != hipSuccess)
     return fail(c, FDMOD_EDEVICE, "model update: %s", hipGetErrorString(e));
   return fdmod_reset(c);
+}
+
+extern "C" int fdmod_batch_gradient(fdmod_batch *b, const float *const *obs_vx, const float *const *obs_vz,
+                                    const float *const *obs_p, const fdmod_grad_opts *opts, const int *x0,
+                                    int nx_full, double *misfit, double *grad)
+{
+  if (!b) return FDMOD_EINVAL;
+  if (!opts || !misfit || !grad) return bfail(b, FDMOD_EINVAL, "fdmod_batch_gradient: NULL opts/misfit/grad");
+  const int n = (int)b->shots.size(), nz = b->p.nz;
+  size_t nout = 0;
+  std::vector<size_t> off(n, 0);
+  for (int s = 0; s < n; s++) {
+    const int nxs = b->shots[s]->p.nx;
+    if (x0 && (x0[s] < 0 || x0[s] + nxs > nx_full))
+      return bfail(b, FDMOD_EINVAL, "shot %d: crop [%d, %d) outside the model (nx = %d)", s, x0[s], x0[s] + nxs,
+                   nx_full);
+    off[s] = nout;
+    if (!x0) nout += 4 * (size_t)nxs * nz;
+  }
+  if (x0) nout = 4 * (size_t)nx_full * nz;
+  double *d_out = nullptr;
+  BHIP(b, hipMalloc(&d_out, nout * sizeof(double)));
+  BHIP(b, hipMemsetAsync(d_out, 0, nout * sizeof(double), b->stream));
+  // shots run in groups whose gradient storage fits in free device memory
+  size_t free_b = 0, total_b = 0;
+  BHIP(b, hipMemGetInfo(&free_b, &total_b));
+  size_t per = 1;
+  for (fdmod_ctx *c : b->shots) per = std::max(per, grad_bytes(c, *opts));
+  const int group = (int)std::max<size_t>(1, std::min<size_t>(n, (size_t)(0.85 * free_b) / per));
+  double Jsum = 0.0;
+  int rc = FDMOD_OK;
+  for (int g0 = 0; g0 < n && rc == FDMOD_OK; g0 += group) {
+    const int g1 = std::min(n, g0 + group);
+    std::vector<GradRun> runs(g1 - g0);
+    int bad = -1;
+    auto check = [&](int s, int r) {
+      if (r && rc == FDMOD_OK) {
+        rc = r;
+        bad = s;
+      }
+    };
+    for (int s = g0; s < g1 && rc == FDMOD_OK; s++)
+      check(s, grad_begin(runs[s - g0], b->shots[s], obs_vx ? obs_vx[s] : nullptr, obs_vz ? obs_vz[s] : nullptr,
+                          obs_p ? obs_p[s] : nullptr, opts));
+    // the shots advance in lockstep, each on its own stream
+    for (int k = 0; k < b->p.nt && rc == FDMOD_OK; k++)
+      for (int s = g0; s < g1; s++) check(s, grad_forward(runs[s - g0], k));
+    for (int s = g0; s < g1 && rc == FDMOD_OK; s++) check(s, grad_residual(runs[s - g0]));
+    while (rc == FDMOD_OK && !runs[0].done)
+      for (int s = g0; s < g1; s++) check(s, grad_backward_next(runs[s - g0]));
+    for (int s = g0; s < g1 && rc == FDMOD_OK; s++) check(s, grad_finish(runs[s - g0]));
+    for (int s = g0; s < g1 && rc == FDMOD_OK; s++) {
+      fdmod_ctx *c = b->shots[s];
+      double J = 0.0;
+      hipError_t e = hipMemcpyAsync(&J, c->d_red + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) {
+        rc = bfail(b, FDMOD_EDEVICE, "shot %d: %s", s, hipGetErrorString(e));
+        break;
+      }
+      Jsum += J;
+      launch_accum_grad(b->stream, c->d_grad, c->p.nx, nz, x0 ? x0[s] : 0, x0 ? nx_full : c->p.nx,
+                        d_out + (x0 ? 0 : off[s]));
+      e = hipStreamSynchronize(b->stream);
+      if (e != hipSuccess) rc = bfail(b, FDMOD_EDEVICE, "gradient sum: %s", hipGetErrorString(e));
+    }
+    if (bad >= 0) bfail(b, rc, "shot %d: %s", bad, b->shots[bad]->err.c_str());
+    for (int s = g0; s < g1; s++) release_grad(b->shots[s]);
+  }
+  if (rc == FDMOD_OK) {
+    hipError_t e = hipMemcpy(grad, d_out, nout * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = bfail(b, FDMOD_EDEVICE, "gradient download: %s", hipGetErrorString(e));
+    *misfit = Jsum;
+  }
+  dfree(d_out);
+  return rc;
 }

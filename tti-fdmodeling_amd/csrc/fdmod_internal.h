@@ -30,6 +30,7 @@ constexpr int kRing = 11;          // LDS ring of staged columns (10 live + 1 in
 constexpr int kFlagW = 11;         // columns per fine flag tile
 constexpr int kDefaultSegLV = 44;  // measured best on a 1256 x 8002 grid
 constexpr int kDefaultSegLS = 11;
+constexpr int kMinTiles = 3000;    // blocks per sweep below which segments shorten
 
 // Device-side geometry of a padded grid.
 struct Grid {
@@ -96,8 +97,28 @@ struct AdjArgs {
   float *mvx, *mvz, *mxx, *mzz, *mxz;  // adjoint state
   const float *coef1, *coef2, *invrho;
   const float *c11, *c13, *c15, *c33, *c35, *c55;
-  const float *fvx, *fvz;              // forward velocity v_{k+1} (imaging only)
+  float *snap[3];                      // V-adjoint: copy of the incoming mu_sigma (or null)
   double *G[6];                        // imaging sums for c11, c13, c15, c33, c35, c55
+};
+
+// One imaging launch over nk <= kImgK steps: mu_sigma snapshots and the stored
+// v_{k+1} of each step (FP32 fv, or FP16 hv with per-(column, block) scales sv).
+constexpr int kImgK = 8;
+struct ImgBlock {
+  int nk;
+  const float *lam[kImgK][3];
+  const float *fv[kImgK][2];
+  const _Float16 *hv[kImgK][2];
+  const float *sv[kImgK][2];
+};
+
+// FP16 packing of nf padded fields: dst = src * 2^(14-E) per (column, 64-row block),
+// inv = 2^(E-14) per block ([nx][ldz/64]); unpacking writes src back.
+struct Pack16 {
+  int nx, ldz, nf;
+  float *src[5];
+  _Float16 *dst[5];
+  float *inv[5];
 };
 
 // ---- launchers (kernels_setup.hip) ----
@@ -133,7 +154,7 @@ void launch_energy_final(hipStream_t s, const float *partial, int nblocks, float
 // ---- launchers (kernels_adjoint.hip) ----
 void launch_adj_ssweep(hipStream_t s, const AdjArgs &a);
 void launch_adj_vsweep(hipStream_t s, const AdjArgs &a);
-void launch_image(hipStream_t s, const AdjArgs &a);
+void launch_image_blk(hipStream_t s, const AdjArgs &a, const ImgBlock &b);
 void launch_adj_fs(hipStream_t s, const AdjArgs &a);
 void launch_adj_inject(hipStream_t s, int nr, int ldz, const int *gx, const int *gz, const float *r1,
                        const float *r2, float *f1, float *f2, float scale);
@@ -144,5 +165,7 @@ void launch_dot(hipStream_t s, const float *a, const float *bf, const double *bd
 void launch_chain_bond(hipStream_t s, const Grid &g, int nx, int nz, int fs_row, const DevArrays &d,
                        const double *G, size_t gstride, double theta_scale, double *grad);
 void launch_rescale_frozen(hipStream_t s, const Grid &g, float *vp, float *vs, float *rho, const float *Beta);
+void launch_pack16(hipStream_t s, const Pack16 &p, bool pack);
+void launch_accum_grad(hipStream_t s, const double *src, int nx, int nz, int x0, int nx_full, double *dst);
 
 }  // namespace fdm

@@ -3,7 +3,7 @@
 //
 // One forward step is u_{k+1} = FS(S(V(u_k) + s^v_k)) + s^sigma_k and d_k = R u_{k+1}.
 // Going backwards through step k the adjoint state mu (same five fields) takes
-//   mu += R^T r_k;  mu <- FS^T mu;  G += mu_sigma (x) E(v_{k+1});
+//   mu += R^T r_k;  mu <- FS^T mu;  G += mu_sigma (x) E(v_{k+1}) (in blocks of steps);
 //   mu_v <- coef1 mu_v + E^T(coef2 C mu_sigma);  mu_sigma <- coef1 mu_sigma + D^T(coef2 invrho mu_v)
 // where D (stress -> velocity, offsets -4..+5) and E (velocity -> strain, -5..+4) are
 // the two RSG derivative operators.  On the grid D^T = -E-shaped and E^T = -D-shaped
@@ -93,12 +93,14 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   const int iz = z0 + lane;
   const bool zok = iz < g.izEnd;
   const int rs = z0 - 4;
-  {
-    MCol c[9];
+  // prologue in groups of three columns (a staged row is 10 floats)
 #pragma unroll
-    for (int m = 0; m < 9; m++) load_mcol(c[m], a, xa - 4 + m, rs, lane, g);
+  for (int m0 = 0; m0 < 9; m0 += 3) {
+    MCol c[3];
 #pragma unroll
-    for (int m = 0; m < 9; m++) store_mcol(ring, m, c[m], lane);
+    for (int m = 0; m < 3; m++) load_mcol(c[m], a, xa - 4 + m0 + m, rs, lane, g);
+#pragma unroll
+    for (int m = 0; m < 3; m++) store_mcol(ring, m0 + m, c[m], lane);
   }
   MCol colq[PD];
   VAdjOps opq[PD];
@@ -196,6 +198,8 @@ struct SAdjOps {
 
 // mu_sigma(q) <- k1(q) mu_sigma(q) - (Dx Nx, Dz Nz, Dz Nx + Dx Nz)(q), with k1 = coef1
 // inside U and 1 on the image rows above a free surface (rows [g.izStart, a.izU)).
+// The incoming mu_sigma is the one the imaging of this step uses; with a.snap set it
+// is also written there (points of U) for the blocked imaging kernel.
 template <int PD>
 __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
 {
@@ -257,6 +261,11 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
         a.mxx[o] = op.k1 * op.s1 - nxdx;
         a.mzz[o] = op.k1 * op.s2 - nzdz;
         a.mxz[o] = op.k1 * op.s3 - (nxdz + nzdx);
+        if (a.snap[0] && inU) {  // mu_sigma as imaged at this step (after FS^T)
+          a.snap[0][o] = op.s1;
+          a.snap[1][o] = op.s2;
+          a.snap[2][o] = op.s3;
+        }
       }
 #pragma unroll
       for (int i = 0; i + 1 < PD; i++) {
@@ -269,74 +278,161 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
   }
 }
 
-// ---------------------------------------------------------------- imaging
-// G_j(q) += (mu_sigma (x) E(v_{k+1}))_j, q in U, for the six Bond stiffness arrays in
-// the order c11, c13, c15, c33, c35, c55.  coef2 is time-independent and multiplies
-// the sums once, in the chain-rule kernel.  Products and sums are in double.
-template <int PD>
-__device__ __forceinline__ void image_tile(const AdjArgs &a, int btz, int btx)
+// ---------------------------------------------------------------- FP16 storage
+// Stored wavefields (checkpoints, the velocity history) in FP16 with one power-of-two
+// scale per field, column and 64-row block: a block with maximum |v| < 2^E is stored
+// as v * 2^(14-E) (exact scaling, |value| < 2^14 < 65504) and the block keeps the
+// inverse scale 2^(E-14).  Arithmetic on the decoded values stays in FP32.
+__device__ __forceinline__ float wave_max(float m)
 {
-  __shared__ float ring[2][kRing][LZ];
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  return m;
+}
+
+// One wave per (column, block); blockIdx.y = field.  Rows [Nz, ldz) hold zeros.
+__global__ __launch_bounds__(256) void k_pack16(Pack16 p)
+{
+  const int lane = threadIdx.x & 63;
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nblk = p.ldz / TZ;
+  if (w >= (long)p.nx * nblk) return;
+  const int f = blockIdx.y;
+  const int ix = (int)(w / nblk), b = (int)(w - (long)ix * nblk);
+  const size_t o = (size_t)ix * p.ldz + (size_t)b * TZ + lane;
+  const float v = p.src[f][o];
+  const float m = wave_max(fabsf(v));
+  int E = 0;
+  if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
+  E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
+  p.dst[f][o] = (_Float16)(v * ldexpf(1.0f, 14 - E));
+  if (lane == 0) p.inv[f][(size_t)ix * nblk + b] = ldexpf(1.0f, E - 14);
+}
+
+__global__ __launch_bounds__(256) void k_unpack16(Pack16 p)
+{
+  const int lane = threadIdx.x & 63;
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nblk = p.ldz / TZ;
+  if (w >= (long)p.nx * nblk) return;
+  const int f = blockIdx.y;
+  const int ix = (int)(w / nblk), b = (int)(w - (long)ix * nblk);
+  const size_t o = (size_t)ix * p.ldz + (size_t)b * TZ + lane;
+  p.src[f][o] = (float)p.dst[f][o] * p.inv[f][(size_t)ix * nblk + b];
+}
+
+void launch_pack16(hipStream_t s, const Pack16 &p, bool pack)
+{
+  const long waves = (long)p.nx * (p.ldz / TZ);
+  dim3 grid((unsigned)((waves + 3) / 4), p.nf);
+  if (pack) hipLaunchKernelGGL(k_pack16, grid, dim3(256), 0, s, p);
+  else hipLaunchKernelGGL(k_unpack16, grid, dim3(256), 0, s, p);
+}
+
+// ---------------------------------------------------------------- imaging
+// G_j(q) += sum over a block of nk steps of (mu_sigma (x) E(v_{k+1}))_j, q in U, for
+// the six Bond stiffness arrays in the order c11, c13, c15, c33, c35, c55.  mu_sigma
+// comes from the snapshots the V-adjoint wrote, v_{k+1} from the stored history.  A
+// tile is one 64-row strip by SEGI columns: per step it stages the SEGI + 9 columns of
+// v its stencils read and adds the products to FP64 register accumulators, so the six
+// FP64 sums are read and written once per block of steps instead of once per step.
+// coef2 is time-independent and multiplies the sums in the chain-rule kernel.
+template <bool F16>
+__device__ __forceinline__ void load_vcol_blk(Col<2> &c, const ImgBlock &b, int j, int gx, int rs, int lane,
+                                              const Grid &g)
+{
+  const size_t base = (size_t)gx * g.ldz;
+  const int r0 = rs + lane, r1 = rs + TZ + lane;
+  if (!F16) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      c.lo[k] = (r0 >= 0 && r0 < g.Nz) ? b.fv[j][k][base + r0] : 0.0f;
+      c.hi[k] = (lane < kHalo && r1 < g.Nz) ? b.fv[j][k][base + r1] : 0.0f;
+    }
+    return;
+  }
+  const size_t sb = (size_t)gx * (g.ldz / TZ);
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    c.lo[k] = (r0 >= 0 && r0 < g.Nz) ? (float)b.hv[j][k][base + r0] * b.sv[j][k][sb + r0 / TZ] : 0.0f;
+    c.hi[k] = (lane < kHalo && r1 < g.Nz) ? (float)b.hv[j][k][base + r1] * b.sv[j][k][sb + r1 / TZ] : 0.0f;
+  }
+}
+
+constexpr int kSegImg = 8;  // columns per imaging tile
+
+template <bool F16>
+__global__ __launch_bounds__(64) void k_image_blk(AdjArgs a, ImgBlock b)
+{
+  constexpr int NS = kSegImg + 2 * kN - 1;  // staged columns
+  __shared__ float sv[2][NS][LZ];
   const Grid &g = a.g;
-  const int segL = g.segLS;
+  int btz, btx;
+  if (!tile_of_block(g, kSegImg, btz, btx)) return;
   const int lane = threadIdx.x;
   const int z0 = g.izStart + btz * TZ;
-  const int xa = g.ixStart + btx * segL;
-  const int nc = min(segL, g.ixEnd - xa);
+  const int xa = g.ixStart + btx * kSegImg;
+  const int nc = min(kSegImg, g.ixEnd - xa);
   const int iz = z0 + lane;
   const bool zok = iz < g.izEnd;
   const int rs = z0 - 5;
-  const float *const fv[2] = {a.fvx, a.fvz};
-  {
-    Col<2> c[9];
-#pragma unroll
-    for (int m = 0; m < 9; m++) load_col<2>(c[m], fv, xa - 5 + m, rs, lane, g);
-#pragma unroll
-    for (int m = 0; m < 9; m++) store_col<2>(ring, m, c[m], lane);
-  }
-  Col<2> colq[PD];
-#pragma unroll
-  for (int i = 0; i < PD; i++)
-    if (i <= nc - 1) load_col<2>(colq[i], fv, xa + 4 + i, rs, lane, g);
   const float invdx = g.invdx, invdz = g.invdz;
-  for (int k0 = 0; k0 < nc; k0 += kRing) {
-    static_for<0, kRing>([&](auto uc) {
-      constexpr int u = decltype(uc)::value;
-      const int k = k0 + u;
-      if (k >= nc) return;
-      const int c = xa + k;
-      Col<2> cn;
-      if (k <= nc - PD - 1) load_col<2>(cn, fv, c + 4 + PD, rs, lane, g);
-      const size_t o = (size_t)c * g.ldz + iz;
-      float lx = 0.0f, lz = 0.0f, lc = 0.0f;
-      if (zok) {
-        lx = a.mxx[o];
-        lz = a.mzz[o];
-        lc = a.mxz[o];
-      }
-      store_col<2>(ring, (u + 9) % kRing, colq[0], lane);
-      __syncthreads();
-      if (zok) {
-        float d1, d2;
-        rsg_bwd([&](int dx, int dz) { return ring[0][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, d1, d2);
-        const float vxdx = 0.5f * (d1 + d2) * invdx;
-        const float vxdz = 0.5f * (d1 - d2) * invdz;
-        rsg_bwd([&](int dx, int dz) { return ring[1][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, d1, d2);
-        const float vzdx = 0.5f * (d1 + d2) * invdx;
-        const float vzdz = 0.5f * (d1 - d2) * invdz;
-        const double ex = vxdx, ez = vzdz, eg = (double)(vzdx + vxdz);
-        const double X = lx, Z = lz, Y = lc;
-        a.G[0][o] += X * ex;
-        a.G[1][o] += X * ez + Z * ex;
-        a.G[2][o] += X * eg + Y * ex;
-        a.G[3][o] += Z * ez;
-        a.G[4][o] += Z * eg + Y * ez;
-        a.G[5][o] += Y * eg;
-      }
+  double acc[kSegImg][6];
 #pragma unroll
-      for (int i = 0; i + 1 < PD; i++) colq[i] = colq[i + 1];
-      colq[PD - 1] = cn;
-    });
+  for (int c = 0; c < kSegImg; c++)
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc[c][q] = 0.0;
+  for (int j = 0; j < b.nk; j++) {
+    float lam[kSegImg][3];
+#pragma unroll
+    for (int c = 0; c < kSegImg; c++) {
+      const size_t o = (size_t)(xa + c) * g.ldz + iz;
+      const bool ok = zok && c < nc;
+#pragma unroll
+      for (int q = 0; q < 3; q++) lam[c][q] = ok ? b.lam[j][q][o] : 0.0f;
+    }
+    Col<2> cs[NS];
+#pragma unroll
+    for (int m = 0; m < NS; m++)
+      if (m < nc + 2 * kN - 1) load_vcol_blk<F16>(cs[m], b, j, xa - kN + m, rs, lane, g);
+#pragma unroll
+    for (int m = 0; m < NS; m++)
+      if (m < nc + 2 * kN - 1) {
+        sv[0][m][lane] = cs[m].lo[0];
+        sv[1][m][lane] = cs[m].lo[1];
+        if (lane < kHalo) {
+          sv[0][m][TZ + lane] = cs[m].hi[0];
+          sv[1][m][TZ + lane] = cs[m].hi[1];
+        }
+      }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kSegImg; c++) {
+      if (c >= nc || !zok) continue;
+      float d1, d2;
+      rsg_bwd([&](int dx, int dz) { return sv[0][c + kN + dx][lane + kN + dz]; }, d1, d2);
+      const float vxdx = 0.5f * (d1 + d2) * invdx;
+      const float vxdz = 0.5f * (d1 - d2) * invdz;
+      rsg_bwd([&](int dx, int dz) { return sv[1][c + kN + dx][lane + kN + dz]; }, d1, d2);
+      const float vzdx = 0.5f * (d1 + d2) * invdx;
+      const float vzdz = 0.5f * (d1 - d2) * invdz;
+      const double ex = vxdx, ez = vzdz, eg = (double)(vzdx + vxdz);
+      const double X = lam[c][0], Z = lam[c][1], Y = lam[c][2];
+      acc[c][0] += X * ex;
+      acc[c][1] += X * ez + Z * ex;
+      acc[c][2] += X * eg + Y * ex;
+      acc[c][3] += Z * ez;
+      acc[c][4] += Z * eg + Y * ez;
+      acc[c][5] += Y * eg;
+    }
+    __syncthreads();
+  }
+  if (!zok) return;
+#pragma unroll
+  for (int c = 0; c < kSegImg; c++) {
+    if (c >= nc) continue;
+    const size_t o = (size_t)(xa + c) * g.ldz + iz;
+#pragma unroll
+    for (int q = 0; q < 6; q++) a.G[q][o] += acc[c][q];
   }
 }
 
@@ -356,13 +452,6 @@ __global__ __launch_bounds__(64) void k_adj_vsweep(AdjArgs a)
   adj_v_tile<kPipeAdj>(a, btz, btx);
 }
 
-__global__ __launch_bounds__(64) void k_image(AdjArgs a)
-{
-  int btz, btx;
-  if (!tile_of_block(a.g, a.g.segLS, btz, btx)) return;
-  image_tile<kPipeAdj>(a, btz, btx);
-}
-
 static dim3 tiles_grid(const Grid &g, int segL)
 {
   const int n = g.tilesZ * ((g.ixEnd - g.ixStart + segL - 1) / segL);
@@ -379,9 +468,10 @@ void launch_adj_vsweep(hipStream_t s, const AdjArgs &a)
   hipLaunchKernelGGL(k_adj_vsweep, tiles_grid(a.g, a.g.segLS), dim3(64), 0, s, a);
 }
 
-void launch_image(hipStream_t s, const AdjArgs &a)
+void launch_image_blk(hipStream_t s, const AdjArgs &a, const ImgBlock &b)
 {
-  hipLaunchKernelGGL(k_image, tiles_grid(a.g, a.g.segLS), dim3(64), 0, s, a);
+  if (b.hv[0][0]) hipLaunchKernelGGL(k_image_blk<true>, tiles_grid(a.g, kSegImg), dim3(64), 0, s, a, b);
+  else hipLaunchKernelGGL(k_image_blk<false>, tiles_grid(a.g, kSegImg), dim3(64), 0, s, a, b);
 }
 
 // ---------------------------------------------------------------- free surface
@@ -667,6 +757,21 @@ void launch_chain_bond(hipStream_t s, const Grid &g, int nx, int nz, int fs_row,
 {
   dim3 grid((nz + 127) / 128, nx);
   hipLaunchKernelGGL(k_chain_bond, grid, dim3(128), 0, s, g, nx, nz, fs_row, d, G, gstride, theta_scale, grad);
+}
+
+// Add one shot's crop gradient [4][nx][nz] into the model gradient [4][nx_full][nz]
+// at column offset x0 (shots are added one after another, in shot order).
+__global__ void k_accum_grad(const double *src, int nx, int nz, int x0, int nx_full, double *dst)
+{
+  const int jz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jx = blockIdx.y, m = blockIdx.z;
+  if (jz >= nz || jx >= nx) return;
+  dst[((size_t)m * nx_full + x0 + jx) * nz + jz] += src[((size_t)m * nx + jx) * nz + jz];
+}
+
+void launch_accum_grad(hipStream_t s, const double *src, int nx, int nz, int x0, int nx_full, double *dst)
+{
+  hipLaunchKernelGGL(k_accum_grad, dim3((nz + 255) / 256, nx, 4), dim3(256), 0, s, src, nx, nz, x0, nx_full, dst);
 }
 
 // Frozen absorbing layers: a new padded model is rescaled by the stored Beta as in

@@ -9,7 +9,7 @@ import ctypes as ct
 
 import numpy as np
 
-from ._lib import AID, FORMS, FdmodError, Params, SNAPSHOT_FN, _f32, _ptr, load
+from ._lib import AID, FORMS, STORAGE, FdmodError, GradOpts, Params, SNAPSHOT_FN, _f32, _ptr, load
 
 __all__ = ["Modeling", "Batch", "FdmodError", "FORMS", "load"]
 
@@ -149,13 +149,16 @@ class Modeling:
             out.append(a)
         return out, [None if a is None else _ptr(a, ct.c_float) for a in out]
 
-    def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, ncheckpoints=0):
+    def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, ncheckpoints=0, storage="f32", history=False):
         """J = 1/2 |d - obs|^2 over the given components and dJ/d(vp, eps, delta, theta[deg])
-        as a float64 array (4, nx, nz); runs the forward from rest (Bond form)."""
+        as a float64 array (4, nx, nz); runs the forward from rest (Bond form).
+        storage "f32" or "f16" (stored wavefields); history keeps every step's velocity
+        instead of checkpoints."""
         keep, ptrs = self._series((obs_vx, obs_vz, obs_p))
         J = ct.c_double(0)
         g = np.empty(4 * self.nx * self.nz, dtype=np.float64)
-        self._check(self.L.fdmod_gradient(self.c, *ptrs, int(ncheckpoints), ct.byref(J), _ptr(g, ct.c_double)))
+        opts = GradOpts(int(ncheckpoints), STORAGE[storage], int(bool(history)))
+        self._check(self.L.fdmod_gradient_opts(self.c, *ptrs, ct.byref(opts), ct.byref(J), _ptr(g, ct.c_double)))
         return J.value, g.reshape(4, self.nx, self.nz)
 
     def adjoint_source(self, y_vx=None, y_vz=None, y_p=None):
@@ -257,3 +260,45 @@ class Batch:
         out = np.empty(self.nr[shot] * self.ntsample[shot], dtype=np.float32)
         self._check(self.L.fdmod_batch_get_traces(self.b, shot, comp, _ptr(out, ct.c_float)))
         return out.reshape(self.nr[shot], self.ntsample[shot])
+
+    def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, ncheckpoints=0, storage="f32", history=False,
+                 x0=None, nx_full=0):
+        """Misfit gradients of all shots (Bond form).  obs_* are lists of per-shot
+        (ntrace, ntsample) arrays (None = component absent).  With x0 (per-shot first
+        model column) the result is the FP64 sum over shots on the nx_full-column model,
+        shape (4, nx_full, nz); otherwise a list of per-shot (4, nx[s], nz) arrays.
+        Returns (total misfit, gradient)."""
+        n = len(self.nxs)
+        keep = []
+
+        def ptrs(series):
+            if series is None:
+                return None
+            arr = (ct.POINTER(ct.c_float) * n)()
+            for s, a in enumerate(series):
+                a = _f32(a).reshape(-1)
+                if a.size != self.nr[s] * self.ntsample[s]:
+                    raise ValueError("shot %d: data series must have ntrace*ntsample samples" % s)
+                keep.append(a)
+                arr[s] = _ptr(a, ct.c_float)
+            return arr
+
+        pv, pz, pp = ptrs(obs_vx), ptrs(obs_vz), ptrs(obs_p)
+        opts = GradOpts(int(ncheckpoints), STORAGE[storage], int(bool(history)))
+        J = ct.c_double(0)
+        if x0 is not None:
+            x0a = np.ascontiguousarray(x0, dtype=np.int32)
+            out = np.empty(4 * int(nx_full) * self.nz, dtype=np.float64)
+            self._check(self.L.fdmod_batch_gradient(self.b, pv, pz, pp, ct.byref(opts), _ptr(x0a, ct.c_int),
+                                                    int(nx_full), ct.byref(J), _ptr(out, ct.c_double)))
+            return J.value, out.reshape(4, int(nx_full), self.nz)
+        tot = sum(4 * int(w) * self.nz for w in self.nxs)
+        out = np.empty(tot, dtype=np.float64)
+        self._check(self.L.fdmod_batch_gradient(self.b, pv, pz, pp, ct.byref(opts), None, 0, ct.byref(J),
+                                                _ptr(out, ct.c_double)))
+        res, o = [], 0
+        for w in self.nxs:
+            k = 4 * int(w) * self.nz
+            res.append(out[o:o + k].reshape(4, int(w), self.nz))
+            o += k
+        return J.value, res

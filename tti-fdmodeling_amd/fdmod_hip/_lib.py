@@ -33,6 +33,12 @@ class Params(ct.Structure):
                 ("fpeak", ct.c_float), ("scalar", ct.c_float), ("mode", ct.c_int)]
 
 
+class GradOpts(ct.Structure):
+    _fields_ = [("ncheckpoints", ct.c_int), ("storage", ct.c_int), ("history", ct.c_int)]
+
+
+STORAGE = {"f32": 0, "f16": 1}
+
 SNAPSHOT_FN = ct.CFUNCTYPE(ct.c_int, ct.c_void_p, ct.c_int, ct.c_char_p, _fp, ct.c_int, ct.c_int)
 
 SIGNATURES = {
@@ -68,6 +74,9 @@ SIGNATURES = {
     "fdmod_batch_get_traces": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _fp]),
     "fdmod_gradient": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp, ct.c_int, _dp, _dp]),
     "fdmod_update_model": (ct.c_int, [ct.c_void_p] + [_fp] * 6),
+    "fdmod_gradient_opts": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp, ct.POINTER(GradOpts), _dp, _dp]),
+    "fdmod_batch_gradient": (ct.c_int, [ct.c_void_p, ct.POINTER(_fp), ct.POINTER(_fp), ct.POINTER(_fp),
+                                        ct.POINTER(GradOpts), _ip, ct.c_int, _dp, _dp]),
     "fdmod_adjoint_source": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp, _dp]),
     "fdmod_adjoint_dot": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp, _dp, _dp]),
 }
