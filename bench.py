@@ -18,6 +18,12 @@ Also reported:
                 a port) on the host cores, on a bounded number of steps of the same
                 workload, plus a bitwise comparison of the GPU wavefield after the
                 same steps
+  batched       64 uneven aperture crops in one batched, graph-captured loop against
+                the same shots run one after another
+  gradients     misfit gradients wrt (vp, eps, delta, theta): one shot on a 1126 x 2000
+                crop (plain FP32 checkpointed, and FP16 strain history), and 32 shots
+                (batched FP16 against the plain single-shot path); the FP64 oracle
+                gradient on a reduced replica is the CPU reference and the error bar
 """
 import argparse
 import json
@@ -173,6 +179,163 @@ def batched_vs_loop(form, m, cfg, steps, nshots=64, seed=3):
             "speedup": round(wl / wb, 3)}
 
 
+GRAD_CFG = dict(nz=1126, dx=10.0, dz=10.0, dt=8e-4, nt=4000, ealThick=60, freesurface=0, fpeak=12.0, sz=2, rz=2)
+
+
+def _grad_shot(cls, m, w, sx, nt, **kw):
+    c = GRAD_CFG
+    g = cls("bond", c["nz"], w, nt, c["dx"], c["dz"], c["dt"], ealThick=c["ealThick"], freesurface=c["freesurface"],
+            fpeak=c["fpeak"], **kw)
+    g.set_model(m["vp"], m["vs"], m["rho"], m["eps"], m["delta"], m["theta"])
+    from fdmod_hip import synth
+
+    g.set_source(1, [sx], [c["sz"]], synth.ricker(c["fpeak"], c["dt"], nt)[None])
+    g.set_receivers(2, 1, np.arange(w), np.full(w, c["rz"]))
+    return g
+
+
+def _rel(a, b):
+    return [float(np.linalg.norm(a[i] - b[i]) / max(np.linalg.norm(b[i]), 1e-300)) for i in range(4)]
+
+
+def gradient_single(full, true_full, nt):
+    """Adjoint gradient workload: one shot on a 1126 x 2000 crop, nt steps; the plain
+    FP32 checkpointed gradient and the FP16 full-history gradient."""
+    from fdmod_hip import Modeling, synth
+
+    W, x0 = 2000, 2936
+    bg, tr = synth.crop(full, x0, W), synth.crop(true_full, x0, W)
+    gt = _grad_shot(Modeling, tr, W, W // 2, nt)
+    gt.run()
+    obs = gt.traces(4).copy()
+    gt.close()
+    g = _grad_shot(Modeling, bg, W, W // 2, nt)
+    out = {"workload": f"one shot, Bond form, 1126 x {W} crop of the seeded model (columns {x0}..{x0 + W - 1}), "
+                       f"true model = background + 5 Gaussian anomalies, nt={nt}, dt=0.8 ms, no free surface, "
+                       f"{W} pressure receivers; gradient wrt vp, eps, delta, theta"}
+    res = {}
+    for name, kw in (("plain_f32_checkpoints", dict()), ("f16_history", dict(storage="f16", history=True))):
+        g.gradient(obs_p=obs, **kw)  # warm-up (allocations)
+        gpu_sync()
+        t0 = time.perf_counter()
+        J, gr = g.gradient(obs_p=obs, **kw)
+        wall = time.perf_counter() - t0
+        res[name] = gr
+        out[name] = {"seconds": round(wall, 4), "gradients_per_hour": round(3600.0 / wall, 1), "misfit": J}
+    out["f16_vs_f32_rel_l2"] = [round(x, 7) for x in _rel(res["f16_history"], res["plain_f32_checkpoints"])]
+    g.close()
+    return out
+
+
+def gradient_batched(full, true_full, nt, nshots, seed=5):
+    """Batched gradients: nshots shots with crops W ~ U{1500..3000} around uniformly
+    drawn sources, FP16 strain history, FP64 sum over shots into the full-model
+    gradient; against the plain FP32 checkpointed single-shot path on the same shots."""
+    from fdmod_hip import Batch, Modeling, synth
+
+    nx_full = full["vp"].shape[0]
+    rng = np.random.default_rng(seed)
+    shots = []
+    for _ in range(nshots):
+        w = int(rng.integers(1500, 3001))
+        sx = int(rng.integers(0, nx_full))
+        x0 = int(min(max(sx - w // 2, 0), nx_full - w))
+        shots.append((x0, w, sx - x0))
+    log(f"bench: observed data for {nshots} shots (batched forward, true model)")
+    kw = dict(ealThick=GRAD_CFG["ealThick"], freesurface=GRAD_CFG["freesurface"], fpeak=GRAD_CFG["fpeak"])
+    c = GRAD_CFG
+    bt = Batch("bond", c["nz"], [w for _, w, _ in shots], nt, c["dx"], c["dz"], c["dt"], **kw)
+    for s, (x0, w, sx) in enumerate(shots):
+        mc = synth.crop(true_full, x0, w)
+        bt.set_model(s, mc["vp"], mc["vs"], mc["rho"], mc["eps"], mc["delta"], mc["theta"])
+        bt.set_source(s, 1, [sx], [c["sz"]], synth.ricker(c["fpeak"], c["dt"], nt)[None])
+        bt.set_receivers(s, 2, 1, np.arange(w), np.full(w, c["rz"]))
+    bt.run()
+    obs = [bt.traces(s, 4).copy() for s in range(nshots)]
+    bt.close()
+    b = Batch("bond", c["nz"], [w for _, w, _ in shots], nt, c["dx"], c["dz"], c["dt"], **kw)
+    for s, (x0, w, sx) in enumerate(shots):
+        mc = synth.crop(full, x0, w)
+        b.set_model(s, mc["vp"], mc["vs"], mc["rho"], mc["eps"], mc["delta"], mc["theta"])
+        b.set_source(s, 1, [sx], [c["sz"]], synth.ricker(c["fpeak"], c["dt"], nt)[None])
+        b.set_receivers(s, 2, 1, np.arange(w), np.full(w, c["rz"]))
+    log(f"bench: batched FP16 gradients of {nshots} shots")
+    gpu_sync()
+    t0 = time.perf_counter()
+    Jb, gb = b.gradient(obs_p=obs, storage="f16", history=True, x0=[x0 for x0, _, _ in shots], nx_full=nx_full)
+    wb = time.perf_counter() - t0
+    b.close()
+    log(f"bench: plain FP32 checkpointed gradients of the same {nshots} shots, one after another")
+    gp = np.zeros_like(gb)
+    wp, Jp = 0.0, 0.0
+    for s, (x0, w, sx) in enumerate(shots):
+        g = _grad_shot(Modeling, synth.crop(full, x0, w), w, sx, nt)
+        gpu_sync()
+        t0 = time.perf_counter()
+        J, gr = g.gradient(obs_p=obs[s])
+        wp += time.perf_counter() - t0
+        gp[:, x0:x0 + w] += gr
+        Jp += J
+        g.close()
+    pts = sum((w + 2 * (c["ealThick"] + 5) - 10) * (c["nz"] + 2 * (c["ealThick"] + 5) - 10) for _, w, _ in shots)
+    return {"workload": f"{nshots} shots, Bond form, crops W ~ U{{1500..3000}} of the seeded 1126 x {nx_full} model "
+                        f"(seed {seed}) around uniformly drawn sources, true model = background + 5 Gaussian "
+                        f"anomalies, nt={nt}, dt=0.8 ms, no free surface, pressure receivers in every crop column",
+            "updated_points_per_step": pts,
+            "batched_f16": {"seconds": round(wb, 3), "gradients_per_hour": round(3600.0 * nshots / wb, 1),
+                            "storage": "FP16 strain history, FP32 checkpoint-free, FP64 imaging and shot sum"},
+            "plain_f32_single_shot": {"seconds": round(wp, 3), "gradients_per_hour": round(3600.0 * nshots / wp, 1),
+                                      "storage": "FP32, ceil(sqrt(nt)) checkpoints with recomputation"},
+            "speedup": round(wp / wb, 3),
+            "misfit_rel_diff": abs(Jb - Jp) / Jp,
+            "batched_vs_plain_rel_l2": [round(x, 7) for x in _rel(gb, gp)]}
+
+
+def cpu_gradient_replica(seed=7):
+    """FP64 oracle gradient (the CPU reference for the adjoint) on a reduced replica of
+    the gradient workload, timed, and the GPU FP32 and FP16 gradients on the same
+    replica compared with it."""
+    from fdmod_hip import Modeling, synth
+    from oracle import Oracle
+
+    nz, W, nt = 141, 250, 800
+    full = synth.model_large(nz=nz, nx=W, seed=seed)
+    # a change the data see within nt steps: +3% vp in the rock, plus two anomalies
+    true = synth.add_anomalies(full, n=2)
+    true["vp"] = np.where(full["vs"] > 0, true["vp"] * 1.03, true["vp"]).astype(np.float32)
+    kw = dict(ealThick=30, freesurface=0, fpeak=12.0)
+
+    def mk(cls, m, **extra):
+        g = cls("bond", nz, W, nt, 10.0, 10.0, 8e-4, **kw, **extra)
+        g.set_model(m["vp"], m["vs"], m["rho"], m["eps"], m["delta"], m["theta"])
+        g.set_source(1, [W // 2], [2], synth.ricker(12.0, 8e-4, nt)[None])
+        g.set_receivers(2, 1, np.arange(W), np.full(W, 2))
+        return g
+
+    ot = mk(Oracle, true, prec="f64")
+    ot.run()
+    obs = ot.traces(4).astype(np.float32)
+    o = mk(Oracle, full, prec="f64")
+    t0 = time.perf_counter()
+    Jr, gr = o.gradient(obs_p=obs)
+    wc = time.perf_counter() - t0
+    # the reference's own FP32 arithmetic (the FP32 oracle) against FP64: the floor the
+    # GPU's FP32 stencils share
+    o32 = mk(Oracle, full, prec="f32")
+    J32, g32 = o32.gradient(obs_p=obs)
+    g = mk(Modeling, full)
+    errs = {"reference_fp32_oracle": [round(x, 7) for x in _rel(g32, gr)]}
+    for name, kwg in (("plain_f32_checkpoints", dict()), ("f16_history", dict(storage="f16", history=True)),
+                      ("f16_checkpoints", dict(storage="f16"))):
+        J, gg = g.gradient(obs_p=obs, **kwg)
+        errs[name] = [round(x, 7) for x in _rel(gg, gr)]
+    return {"sample": f"FP64 oracle gradient (serial adjoint) on a {nz} x {W} replica, nt={nt}, ealThick=30, "
+                      "Bond form, no free surface; data from the FP64 oracle on the model with +3% rock vp and "
+                      "two anomalies", "seconds": round(wc, 3),
+            "gradients_per_hour": round(3600.0 / wc, 2), "kind": "port", "cores": 1,
+            "gpu_rel_l2_vs_fp64": errs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,11 +346,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--grad-shots", type=int, default=32)
+    ap.add_argument("--grad-steps", type=int, default=4000)
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--grad-only", action="store_true", help="only the single-shot gradient workload (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU shot sharding is a design note only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.grad_only:
+        from fdmod_hip import synth
+
+        log("bench: gradient workload only")
+        full = synth.model_large()
+        print(json.dumps({"gradients": {"single_shot": gradient_single(full, synth.add_anomalies(full),
+                                                                       args.grad_steps)}}))
+        return
     nt = max(args.steps, args.cpu_steps, 1)
     log(f"bench: building the 1126x7872 workload (nt={nt})")
     cfg, m, src = workload(nt)
@@ -218,6 +393,16 @@ def main():
     batched = None
     if args.batch_steps > 0:
         batched = batched_vs_loop(args.form, m, cfg, args.batch_steps)
+
+    gradients = None
+    if not args.no_grad:
+        from fdmod_hip import synth
+
+        log("bench: gradient workloads (true model = background + 5 anomalies)")
+        true_full = synth.add_anomalies(m)
+        gradients = {"single_shot": gradient_single(m, true_full, args.grad_steps)}
+        if args.grad_shots > 0:
+            gradients["batched"] = gradient_batched(m, true_full, args.grad_steps, args.grad_shots)
 
     forms = {}
     if not args.no_forms:
@@ -254,6 +439,9 @@ def main():
             _, pc = cpu_baseline(cfg, m, src, "coord", n2, fc)
             pc["finite"] = coord_finite
             parity["coord"] = pc
+        if gradients is not None:
+            log("bench: FP64 oracle gradient on a reduced replica")
+            gradients["cpu_fp64_reference"] = cpu_gradient_replica()
 
     out = {
         "metric": "Gpts/s/step batched vs single-shot; gradients/hr at <=1e-3 rel err vs FP64",
@@ -276,6 +464,7 @@ def main():
         "roofline": roofline,
         "forms_gpts": forms,
         "batched": batched,
+        "gradients": gradients,
         "cpu_baseline": cpu,
         "parity_full_size": parity,
     }

@@ -93,8 +93,8 @@ def test_gradient_independent_of_checkpoint_count():
 
 
 def test_gradient_storage_modes():
-    # FP32 full history = checkpointed recomputation bit for bit; FP16 storage (history
-    # or checkpoints, per-block scales) within 1e-3 of the FP64 oracle
+    # FP32 full history = checkpointed recomputation up to the grouping of FP64 sums;
+    # FP16 storage (per-block scales) within 1e-3 of the FP64 oracle
     m, obs = _obs(1, 2)
     g = _mk(Modeling, m, 1, seis=2)
     ref = g.gradient(*obs)
@@ -102,11 +102,11 @@ def test_gradient_storage_modes():
     assert J == ref[0] and all(rel_l2(grad[i], ref[1][i]) <= 1e-12 for i in range(4))
     o = _mk(Oracle, m, 1, seis=2, prec="f64")
     Jr, gr = o.gradient(*obs)
-    for hist in (True, False):
-        J, grad = g.gradient(*obs, storage="f16", history=hist)
+    for storage, hist in (("f16", True), ("f16", False), ("f16all", False)):
+        J, grad = g.gradient(*obs, storage=storage, history=hist)
         assert J == ref[0]  # the forward itself is FP32 in every mode
         errs = [rel_l2(grad[i], gr[i]) for i in range(4)]
-        print("fp16 history" if hist else "fp16 checkpoints", ["%.2e" % e for e in errs], flush=True)
+        print(storage, "history" if hist else "checkpoints", ["%.2e" % e for e in errs], flush=True)
         assert max(errs) <= 1e-3, errs
 
 

@@ -100,3 +100,20 @@ def test_batch_gradient_equals_single_shot_gradients(storage, history):
         Jsum += Js
     assert J == Jsum and J2 == Jsum
     assert np.array_equal(full, ref)
+
+
+def test_batch_gradient_groups_of_one(monkeypatch):
+    # one shot at a time: the pooled gradient memory is lent to shots of different
+    # widths in turn and must grow to each
+    monkeypatch.setenv("FDMOD_GRAD_GROUP", "1")
+    shots = _shots()
+    b = _batch("bond", shots)
+    ns = (NT - 1) // 2 + 1
+    rng = np.random.default_rng(12)
+    obs_p = [rng.standard_normal((len(sh["rx"]), ns)).astype(np.float32) * 1e-6 for sh in shots]
+    for _ in range(2):
+        J, per = b.gradient(obs_p=obs_p, storage="f16", history=True)
+        for s, sh in enumerate(shots):
+            g = _single(Modeling, "bond", sh)
+            Js, gs = g.gradient(obs_p=obs_p[s], storage="f16", history=True)
+            assert np.array_equal(per[s], gs), s

@@ -425,7 +425,7 @@ extern "C" int fdmod_set_energy(fdmod_ctx *c, int on)
 
 SweepArgs sweep_args(fdmod_ctx *c)
 {
-  SweepArgs a;
+  SweepArgs a{};
   float *const *d = c->d.a;
   a.g = c->g;
   a.vx = d[FDMOD_A_VX]; a.vz = d[FDMOD_A_VZ];
@@ -660,6 +660,10 @@ extern "C" void fdmod_batch_destroy(fdmod_batch *b)
   if (!b) return;
   batch_unprepare(b);
   dfree(b->d_step);
+  for (GradMem &m : b->gmem) {
+    dfree(m.mu); dfree(m.G); dfree(m.snap);
+    dfree(m.ckpt.data); dfree(m.ckpt.scale); dfree(m.hist.data); dfree(m.hist.scale);
+  }
   for (fdmod_ctx *c : b->shots) fdmod_destroy(c);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);

@@ -49,6 +49,7 @@ struct fdmod_ctx {
   float *d_mu = nullptr;       // five adjoint fields, each Nx*ldz
   double *d_G = nullptr;       // six imaging sums, each Nx*ldz
   float *d_snap = nullptr;     // mu_sigma snapshots of one imaging block (kImgK x 3 fields)
+  size_t d_mu_n = 0;           // field size (floats) d_mu, d_G and d_snap were allocated for
   FieldStore ckpt;             // checkpointed states (5 fields per slot)
   FieldStore hist;             // v_{k+1} per step (2 fields per slot): one segment or the whole run
   float *d_res = nullptr;      // data-space adjoint sources [3][ntsample][nr]
@@ -69,6 +70,16 @@ int fail(fdmod_ctx *c, int code, const char *fmt, ...);
 
 void dfree(void *p);  // hipFree that ignores null
 
+// Gradient memory of one shot, pooled by a batch and lent to the shot being worked on
+// (re-allocating tens of GB for every group of shots costs more than the gradient).
+struct GradMem {
+  float *mu = nullptr;
+  double *G = nullptr;
+  float *snap = nullptr;
+  size_t n = 0;  // field size the three buffers above were allocated for
+  FieldStore ckpt, hist;
+};
+
 // A batch of shots (fdmod_api.hip: batched forward loop; fdmod_grad.hip: batched
 // gradients).  Each shot is a full modelling context with its own stream.
 struct fdmod_batch {
@@ -88,6 +99,7 @@ struct fdmod_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int chunk = 32;
+  std::vector<GradMem> gmem;  // gradient memory pool, one entry per shot of a group
   std::string err;
 };
 

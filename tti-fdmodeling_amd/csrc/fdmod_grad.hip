@@ -4,18 +4,21 @@
 //
 // Gradient schedule:
 //   1. forward from rest, recording traces; with checkpoints every seg-th state
-//      (5 fields) is stored, with the full history v_{k+1} of every step;
+//      (5 fields) is stored, with the full history the strain E(v_{k+1}) of every
+//      step, written by the stress sweep itself;
 //   2. residuals r = d - obs and J = sum r^2 / 2, reduced in FP64 on the device;
 //   3. for each checkpoint segment, last first: restore its state and recompute its
-//      steps storing v_{k+1} of each (skipped with the full history), then run the
-//      adjoint steps of the segment backwards, imaging with the stored velocities;
+//      steps storing the strain of each (skipped with the full history), then run
+//      the adjoint steps of the segment backwards, imaging with the stored strain;
 //   4. chain rule from the six stiffness sums to (vp, eps, delta, theta) and the
 //      transpose of the edge padding.
 // Stored fields are FP32 copies or FP16 with per-(column, 64-row block) scales.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "fdmod_ctx.h"
@@ -42,7 +45,13 @@ int check_ready(fdmod_ctx *c, const char *what)
 int workspace(fdmod_ctx *c, bool need_G)
 {
   const size_t n = field_floats(c);
+  if (c->d_mu_n < n) {  // (re)size the per-field workspace, which a batch may lend
+    dfree(c->d_mu); dfree(c->d_G); dfree(c->d_snap);
+    c->d_mu = nullptr; c->d_G = nullptr; c->d_snap = nullptr;
+    c->d_mu_n = 0;
+  }
   if (!c->d_mu) HIPCALL(c, hipMalloc(&c->d_mu, kNState * n * sizeof(float)));
+  c->d_mu_n = n;
   if (!c->d_red) HIPCALL(c, hipMalloc(&c->d_red, (kRedBlocks + 8) * sizeof(double)));
   if (need_G && !c->d_G) HIPCALL(c, hipMalloc(&c->d_G, 6 * n * sizeof(double)));
   if (need_G && !c->d_snap) HIPCALL(c, hipMalloc(&c->d_snap, (size_t)kImgK * 3 * n * sizeof(float)));
@@ -102,24 +111,42 @@ int store_io(fdmod_ctx *c, FieldStore &st, size_t slot, int nf, float *const *fi
   return FDMOD_OK;
 }
 
-// Imaging block entry j: snapshot slot j and history slot `slot` (fields vx, vz).
+// History slots hold the strain (vxdx, vzdz, vzdx + vxdz) of a step, 3 fields.
+constexpr int kNStrain = 3;
+
+// Point the stress sweep's strain output at history slot `slot`.
+SweepArgs with_strain(fdmod_ctx *c, const SweepArgs &sa, size_t slot, bool f16)
+{
+  SweepArgs r = sa;
+  const size_t n = field_floats(c), ns = scales_per_field(c);
+  r.estore = f16 ? 2 : 1;
+  for (int f = 0; f < kNStrain; f++) {
+    if (f16) {
+      r.e16[f] = (_Float16 *)c->hist.data + (slot * kNStrain + f) * n;
+      r.esc[f] = c->hist.scale + (slot * kNStrain + f) * ns;
+    } else {
+      r.e32[f] = (float *)c->hist.data + (slot * kNStrain + f) * n;
+    }
+  }
+  return r;
+}
+
+// Imaging block entry j: snapshot slot j and history slot `slot`.
 void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
 {
-  const size_t n = field_floats(c);
+  const size_t n = field_floats(c), ns = scales_per_field(c);
   for (int q = 0; q < 3; q++) b.lam[j][q] = c->d_snap + ((size_t)j * 3 + q) * n;
-  const FieldStore &st = c->hist;
-  if (!f16) {
-    b.fv[j][0] = (const float *)st.data + slot * 2 * n;
-    b.fv[j][1] = b.fv[j][0] + n;
-    b.hv[j][0] = b.hv[j][1] = nullptr;
-    return;
+  for (int f = 0; f < kNStrain; f++) {
+    if (f16) {
+      b.e16[j][f] = (const _Float16 *)c->hist.data + (slot * kNStrain + f) * n;
+      b.esc[j][f] = c->hist.scale + (slot * kNStrain + f) * ns;
+      b.e32[j][f] = nullptr;
+    } else {
+      b.e32[j][f] = (const float *)c->hist.data + (slot * kNStrain + f) * n;
+      b.e16[j][f] = nullptr;
+      b.esc[j][f] = nullptr;
+    }
   }
-  const size_t ns = scales_per_field(c);
-  b.hv[j][0] = (const _Float16 *)st.data + slot * 2 * n;
-  b.hv[j][1] = b.hv[j][0] + n;
-  b.sv[j][0] = st.scale + slot * 2 * ns;
-  b.sv[j][1] = b.sv[j][0] + ns;
-  b.fv[j][0] = b.fv[j][1] = nullptr;
 }
 
 AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
@@ -229,11 +256,6 @@ int state_io(fdmod_ctx *c, size_t slot, bool f16, bool save)
   return store_io(c, c->ckpt, slot, kNState, f, f16, save);
 }
 
-int velocity_io(fdmod_ctx *c, size_t slot, bool f16)
-{
-  float *f[2] = {c->d.a[FDMOD_A_VX], c->d.a[FDMOD_A_VZ]};
-  return store_io(c, c->hist, slot, 2, f, f16, true);
-}
 
 // Run the whole adjoint loop with the given data-space sources (no imaging).
 int adjoint_only(fdmod_ctx *c, const DataSrc &ds, double *srcadj)
@@ -258,7 +280,7 @@ int reduce_to_host(fdmod_ctx *c, double *out)
 // launches of several shots (each on its own stream).
 struct GradRun {
   fdmod_ctx *c = nullptr;
-  bool f16 = false, history = false;
+  bool f16 = false, ck16 = false, history = false;  // FP16 strain / FP16 checkpoints
   int ncp = 1, seg = 1;
   DataSrc ds;
   SweepArgs sa;
@@ -271,14 +293,15 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
                const fdmod_grad_opts *opts)
 {
   if (!opts) return fail(c, FDMOD_EINVAL, "fdmod_gradient: opts must not be NULL");
-  if (opts->storage != FDMOD_STORE_F32 && opts->storage != FDMOD_STORE_F16)
-    return fail(c, FDMOD_EINVAL, "fdmod_gradient: storage must be FDMOD_STORE_F32 or FDMOD_STORE_F16");
+  if (opts->storage < FDMOD_STORE_F32 || opts->storage > FDMOD_STORE_F16_ALL)
+    return fail(c, FDMOD_EINVAL, "fdmod_gradient: storage must be FDMOD_STORE_F32, _F16 or _F16_ALL");
   int rc = check_ready(c, "fdmod_gradient");
   if (rc) return rc;
   const fdmod_params &p = c->p;
   r = GradRun{};
   r.c = c;
-  r.f16 = opts->storage == FDMOD_STORE_F16;
+  r.f16 = opts->storage != FDMOD_STORE_F32;
+  r.ck16 = opts->storage == FDMOD_STORE_F16_ALL;
   r.history = opts->history != 0;
   if ((rc = workspace(c, true))) return rc;
   if ((rc = upload_data(c, obs_vx, obs_vz, obs_p, r.ds))) return rc;
@@ -291,10 +314,10 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   if (r.history) {
     r.seg = p.nt;
     r.ncp = 1;
-    if ((rc = store_reserve(c, c->hist, p.nt, 2, r.f16))) return rc;
+    if ((rc = store_reserve(c, c->hist, p.nt, kNStrain, r.f16))) return rc;
   } else {
-    if ((rc = store_reserve(c, c->ckpt, r.ncp, kNState, r.f16))) return rc;
-    if ((rc = store_reserve(c, c->hist, r.seg, 2, r.f16))) return rc;
+    if ((rc = store_reserve(c, c->ckpt, r.ncp, kNState, r.ck16))) return rc;
+    if ((rc = store_reserve(c, c->hist, r.seg, kNStrain, r.f16))) return rc;
   }
   if ((rc = fdmod_reset(c))) return rc;
   r.sa = sweep_args(c);
@@ -307,10 +330,10 @@ int grad_forward(GradRun &r, int k)
   fdmod_ctx *c = r.c;
   const fdmod_params &p = c->p;
   int rc;
-  if (!r.history && k % r.seg == 0 && (rc = state_io(c, k / r.seg, r.f16, true))) return rc;
-  if ((rc = step(c, p.mode == 1 ? k : p.nt - 1 - k, r.sa, true))) return rc;
-  if (r.history && (rc = velocity_io(c, k, r.f16))) return rc;
-  return FDMOD_OK;
+  if (!r.history && k % r.seg == 0 && (rc = state_io(c, k / r.seg, r.ck16, true))) return rc;
+  const int it = p.mode == 1 ? k : p.nt - 1 - k;
+  if (r.history) return step(c, it, with_strain(c, r.sa, k, r.f16), true);
+  return step(c, it, r.sa, true);
 }
 
 // Residuals (in place over the uploaded observations); J accumulates on the device.
@@ -341,9 +364,9 @@ int grad_backward_next(GradRun &r)
   const int k0 = r.sgi * r.seg, k1 = std::min(p.nt, k0 + r.seg);
   int rc;
   if (r.phase == 0) {
-    if (r.k == k0 && (rc = state_io(c, r.sgi, r.f16, false))) return rc;
-    if ((rc = step(c, p.mode == 1 ? r.k : p.nt - 1 - r.k, r.sa, false))) return rc;
-    if ((rc = velocity_io(c, r.k - k0, r.f16))) return rc;
+    if (r.k == k0 && (rc = state_io(c, r.sgi, r.ck16, false))) return rc;
+    if ((rc = step(c, p.mode == 1 ? r.k : p.nt - 1 - r.k, with_strain(c, r.sa, r.k - k0, r.f16), false)))
+      return rc;
     if (++r.k == k1) {
       r.phase = 1;
       r.k = k1 - 1;
@@ -353,7 +376,7 @@ int grad_backward_next(GradRun &r)
   adj_step(c, r.k, r.ds, r.blk.nk, nullptr);
   add_to_block(c, r.blk, r.blk.nk, r.k - k0, r.f16);
   if (++r.blk.nk == kImgK || r.k == k0) {
-    launch_image_blk(c->stream, adj_args(c, false), r.blk);
+    launch_image(c->stream, adj_args(c, false), r.blk);
     r.blk = ImgBlock{};
   }
   if (r.k > k0) {
@@ -368,29 +391,37 @@ int grad_backward_next(GradRun &r)
   return FDMOD_OK;
 }
 
-// Release the gradient storage and workspace of a context.
-void release_grad(fdmod_ctx *c)
+// Exchange a context's gradient memory with a pool entry.
+void swap_grad_mem(fdmod_ctx *c, GradMem &m)
 {
-  dfree(c->d_mu); dfree(c->d_G); dfree(c->d_snap);
-  dfree(c->ckpt.data); dfree(c->ckpt.scale); dfree(c->hist.data); dfree(c->hist.scale);
-  c->d_mu = nullptr; c->d_G = nullptr; c->d_snap = nullptr;
-  c->ckpt = FieldStore{};
-  c->hist = FieldStore{};
+  std::swap(c->d_mu, m.mu);
+  std::swap(c->d_G, m.G);
+  std::swap(c->d_snap, m.snap);
+  std::swap(c->d_mu_n, m.n);
+  std::swap(c->ckpt, m.ckpt);
+  std::swap(c->hist, m.hist);
+}
+
+size_t held_bytes(const GradMem &m, size_t n)
+{
+  size_t b = m.ckpt.cap_data + m.ckpt.cap_scale + m.hist.cap_data + m.hist.cap_scale;
+  if (m.mu) b += n * (kNState * 4 + 6 * 8 + kImgK * 3 * 4);
+  return b;
 }
 
 // Device bytes one shot's gradient needs beyond its modelling arrays.
 size_t grad_bytes(const fdmod_ctx *c, const fdmod_grad_opts &o)
 {
   const size_t n = field_floats(c), nt = c->p.nt;
-  const size_t esz = o.storage == FDMOD_STORE_F16 ? 2 : 4;
+  const size_t esz = o.storage == FDMOD_STORE_F32 ? 4 : 2, cksz = o.storage == FDMOD_STORE_F16_ALL ? 2 : 4;
   size_t b = n * (kNState * 4 + 6 * 8 + kImgK * 3 * 4);
   if (o.history) {
-    b += nt * 2 * n * esz;
+    b += nt * kNStrain * n * esz;
   } else {
     int ncp = o.ncheckpoints > 0 ? o.ncheckpoints : (int)ceil(sqrt((double)nt));
     ncp = std::min(std::max(ncp, 1), (int)nt);
     const size_t seg = (nt + ncp - 1) / ncp;
-    b += ((nt + seg - 1) / seg * kNState + seg * 2) * n * esz;
+    b += (nt + seg - 1) / seg * kNState * n * cksz + seg * kNStrain * n * esz;
   }
   return b + b / 16;  // block scales and slack
 }
@@ -553,9 +584,15 @@ extern "C" int fdmod_batch_gradient(fdmod_batch *b, const float *const *obs_vx, 
   // shots run in groups whose gradient storage fits in free device memory
   size_t free_b = 0, total_b = 0;
   BHIP(b, hipMemGetInfo(&free_b, &total_b));
-  size_t per = 1;
-  for (fdmod_ctx *c : b->shots) per = std::max(per, grad_bytes(c, *opts));
-  const int group = (int)std::max<size_t>(1, std::min<size_t>(n, (size_t)(0.85 * free_b) / per));
+  size_t per = 1, nmax = 1;
+  for (fdmod_ctx *c : b->shots) {
+    per = std::max(per, grad_bytes(c, *opts));
+    nmax = std::max(nmax, field_floats(c));
+  }
+  for (const GradMem &m : b->gmem) free_b += held_bytes(m, nmax);  // pooled memory is reusable
+  int group = (int)std::max<size_t>(1, std::min<size_t>(n, (size_t)(0.85 * free_b) / per));
+  if (const char *e = getenv("FDMOD_GRAD_GROUP")) group = std::max(1, std::min(group, atoi(e)));  // tuning cap
+  if ((int)b->gmem.size() < group) b->gmem.resize(group);
   double Jsum = 0.0;
   int rc = FDMOD_OK;
   for (int g0 = 0; g0 < n && rc == FDMOD_OK; g0 += group) {
@@ -568,6 +605,7 @@ extern "C" int fdmod_batch_gradient(fdmod_batch *b, const float *const *obs_vx, 
         bad = s;
       }
     };
+    for (int s = g0; s < g1; s++) swap_grad_mem(b->shots[s], b->gmem[s - g0]);
     for (int s = g0; s < g1 && rc == FDMOD_OK; s++)
       check(s, grad_begin(runs[s - g0], b->shots[s], obs_vx ? obs_vx[s] : nullptr, obs_vz ? obs_vz[s] : nullptr,
                           obs_p ? obs_p[s] : nullptr, opts));
@@ -594,7 +632,10 @@ extern "C" int fdmod_batch_gradient(fdmod_batch *b, const float *const *obs_vx, 
       if (e != hipSuccess) rc = bfail(b, FDMOD_EDEVICE, "gradient sum: %s", hipGetErrorString(e));
     }
     if (bad >= 0) bfail(b, rc, "shot %d: %s", bad, b->shots[bad]->err.c_str());
-    for (int s = g0; s < g1; s++) release_grad(b->shots[s]);
+    for (int s = g0; s < g1; s++) {
+      (void)hipStreamSynchronize(b->shots[s]->stream);
+      swap_grad_mem(b->shots[s], b->gmem[s - g0]);
+    }
   }
   if (rc == FDMOD_OK) {
     hipError_t e = hipMemcpy(grad, d_out, nout * sizeof(double), hipMemcpyDeviceToHost);

@@ -57,6 +57,13 @@ struct SweepArgs {
   const float *sin2, *cos2, *sin2t, *sn, *cs, *snv, *csv;
   const uint8_t *undamped;  // per tile: coef1 == 1 and coef2 == dt everywhere in the tile
   int freesurface;
+  // Strain output of the stress sweep for the gradient (Bond): (vxdx, vzdz, vzdx + vxdz)
+  // at every updated point, as FP32 (estore 1) or as FP16 with one power-of-two scale
+  // per field, column and 64-row strip of the sweep (estore 2, scales [Nx][tilesZ]).
+  int estore;
+  float *e32[3];
+  _Float16 *e16[3];
+  float *esc[3];
 };
 
 // Source injection table entry: offset from the source cell and the two
@@ -101,15 +108,16 @@ struct AdjArgs {
   double *G[6];                        // imaging sums for c11, c13, c15, c33, c35, c55
 };
 
-// One imaging launch over nk <= kImgK steps: mu_sigma snapshots and the stored
-// v_{k+1} of each step (FP32 fv, or FP16 hv with per-(column, block) scales sv).
+// One imaging launch over nk <= kImgK steps: the mu_sigma snapshots and the strain
+// E(v_{k+1}) the stress sweep stored for each step (FP32 e32, or FP16 e16 with the
+// per-(column, strip) scales esc).
 constexpr int kImgK = 8;
 struct ImgBlock {
   int nk;
   const float *lam[kImgK][3];
-  const float *fv[kImgK][2];
-  const _Float16 *hv[kImgK][2];
-  const float *sv[kImgK][2];
+  const float *e32[kImgK][3];
+  const _Float16 *e16[kImgK][3];
+  const float *esc[kImgK][3];
 };
 
 // FP16 packing of nf padded fields: dst = src * 2^(14-E) per (column, 64-row block),
@@ -154,7 +162,7 @@ void launch_energy_final(hipStream_t s, const float *partial, int nblocks, float
 // ---- launchers (kernels_adjoint.hip) ----
 void launch_adj_ssweep(hipStream_t s, const AdjArgs &a);
 void launch_adj_vsweep(hipStream_t s, const AdjArgs &a);
-void launch_image_blk(hipStream_t s, const AdjArgs &a, const ImgBlock &b);
+void launch_image(hipStream_t s, const AdjArgs &a, const ImgBlock &b);
 void launch_adj_fs(hipStream_t s, const AdjArgs &a);
 void launch_adj_inject(hipStream_t s, int nr, int ldz, const int *gx, const int *gz, const float *r1,
                        const float *r2, float *f1, float *f2, float scale);

@@ -330,110 +330,45 @@ void launch_pack16(hipStream_t s, const Pack16 &p, bool pack)
 
 // ---------------------------------------------------------------- imaging
 // G_j(q) += sum over a block of nk steps of (mu_sigma (x) E(v_{k+1}))_j, q in U, for
-// the six Bond stiffness arrays in the order c11, c13, c15, c33, c35, c55.  mu_sigma
-// comes from the snapshots the V-adjoint wrote, v_{k+1} from the stored history.  A
-// tile is one 64-row strip by SEGI columns: per step it stages the SEGI + 9 columns of
-// v its stencils read and adds the products to FP64 register accumulators, so the six
-// FP64 sums are read and written once per block of steps instead of once per step.
-// coef2 is time-independent and multiplies the sums in the chain-rule kernel.
+// the six Bond stiffness arrays in the order c11, c13, c15, c33, c35, c55, where
+// E = (vxdx, vzdz, vzdx + vxdz) is the strain the forward stress sweep computed at q
+// and stored, and mu_sigma the snapshot the V-adjoint wrote.  Pointwise: one thread
+// per point keeps the six sums of the block in FP64 registers, so the FP64 arrays are
+// read and written once per block of steps.  coef2 is time-independent and
+// multiplies the sums in the chain-rule kernel.
 template <bool F16>
-__device__ __forceinline__ void load_vcol_blk(Col<2> &c, const ImgBlock &b, int j, int gx, int rs, int lane,
-                                              const Grid &g)
+__global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
 {
-  const size_t base = (size_t)gx * g.ldz;
-  const int r0 = rs + lane, r1 = rs + TZ + lane;
-  if (!F16) {
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-      c.lo[k] = (r0 >= 0 && r0 < g.Nz) ? b.fv[j][k][base + r0] : 0.0f;
-      c.hi[k] = (lane < kHalo && r1 < g.Nz) ? b.fv[j][k][base + r1] : 0.0f;
-    }
-    return;
-  }
-  const size_t sb = (size_t)gx * (g.ldz / TZ);
-#pragma unroll
-  for (int k = 0; k < 2; k++) {
-    c.lo[k] = (r0 >= 0 && r0 < g.Nz) ? (float)b.hv[j][k][base + r0] * b.sv[j][k][sb + r0 / TZ] : 0.0f;
-    c.hi[k] = (lane < kHalo && r1 < g.Nz) ? (float)b.hv[j][k][base + r1] * b.sv[j][k][sb + r1 / TZ] : 0.0f;
-  }
-}
-
-constexpr int kSegImg = 8;  // columns per imaging tile
-
-template <bool F16>
-__global__ __launch_bounds__(64) void k_image_blk(AdjArgs a, ImgBlock b)
-{
-  constexpr int NS = kSegImg + 2 * kN - 1;  // staged columns
-  __shared__ float sv[2][NS][LZ];
   const Grid &g = a.g;
-  int btz, btx;
-  if (!tile_of_block(g, kSegImg, btz, btx)) return;
-  const int lane = threadIdx.x;
-  const int z0 = g.izStart + btz * TZ;
-  const int xa = g.ixStart + btx * kSegImg;
-  const int nc = min(kSegImg, g.ixEnd - xa);
-  const int iz = z0 + lane;
-  const bool zok = iz < g.izEnd;
-  const int rs = z0 - 5;
-  const float invdx = g.invdx, invdz = g.invdz;
-  double acc[kSegImg][6];
+  const int iz = g.izStart + blockIdx.x * blockDim.x + threadIdx.x;
+  const int ix = g.ixStart + blockIdx.y;
+  if (iz >= g.izEnd) return;
+  const size_t o = (size_t)ix * g.ldz + iz;
+  const size_t so = (size_t)ix * g.tilesZ + (iz - g.izStart) / TZ;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int c = 0; c < kSegImg; c++)
-#pragma unroll
-    for (int q = 0; q < 6; q++) acc[c][q] = 0.0;
-  for (int j = 0; j < b.nk; j++) {
-    float lam[kSegImg][3];
-#pragma unroll
-    for (int c = 0; c < kSegImg; c++) {
-      const size_t o = (size_t)(xa + c) * g.ldz + iz;
-      const bool ok = zok && c < nc;
-#pragma unroll
-      for (int q = 0; q < 3; q++) lam[c][q] = ok ? b.lam[j][q][o] : 0.0f;
+  for (int j = 0; j < kImgK; j++) {
+    if (j >= b.nk) break;
+    const double X = b.lam[j][0][o], Z = b.lam[j][1][o], Y = b.lam[j][2][o];
+    double ex, ez, eg;
+    if (F16) {
+      ex = (float)b.e16[j][0][o] * b.esc[j][0][so];
+      ez = (float)b.e16[j][1][o] * b.esc[j][1][so];
+      eg = (float)b.e16[j][2][o] * b.esc[j][2][so];
+    } else {
+      ex = b.e32[j][0][o];
+      ez = b.e32[j][1][o];
+      eg = b.e32[j][2][o];
     }
-    Col<2> cs[NS];
-#pragma unroll
-    for (int m = 0; m < NS; m++)
-      if (m < nc + 2 * kN - 1) load_vcol_blk<F16>(cs[m], b, j, xa - kN + m, rs, lane, g);
-#pragma unroll
-    for (int m = 0; m < NS; m++)
-      if (m < nc + 2 * kN - 1) {
-        sv[0][m][lane] = cs[m].lo[0];
-        sv[1][m][lane] = cs[m].lo[1];
-        if (lane < kHalo) {
-          sv[0][m][TZ + lane] = cs[m].hi[0];
-          sv[1][m][TZ + lane] = cs[m].hi[1];
-        }
-      }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < kSegImg; c++) {
-      if (c >= nc || !zok) continue;
-      float d1, d2;
-      rsg_bwd([&](int dx, int dz) { return sv[0][c + kN + dx][lane + kN + dz]; }, d1, d2);
-      const float vxdx = 0.5f * (d1 + d2) * invdx;
-      const float vxdz = 0.5f * (d1 - d2) * invdz;
-      rsg_bwd([&](int dx, int dz) { return sv[1][c + kN + dx][lane + kN + dz]; }, d1, d2);
-      const float vzdx = 0.5f * (d1 + d2) * invdx;
-      const float vzdz = 0.5f * (d1 - d2) * invdz;
-      const double ex = vxdx, ez = vzdz, eg = (double)(vzdx + vxdz);
-      const double X = lam[c][0], Z = lam[c][1], Y = lam[c][2];
-      acc[c][0] += X * ex;
-      acc[c][1] += X * ez + Z * ex;
-      acc[c][2] += X * eg + Y * ex;
-      acc[c][3] += Z * ez;
-      acc[c][4] += Z * eg + Y * ez;
-      acc[c][5] += Y * eg;
-    }
-    __syncthreads();
+    acc[0] += X * ex;
+    acc[1] += X * ez + Z * ex;
+    acc[2] += X * eg + Y * ex;
+    acc[3] += Z * ez;
+    acc[4] += Z * eg + Y * ez;
+    acc[5] += Y * eg;
   }
-  if (!zok) return;
 #pragma unroll
-  for (int c = 0; c < kSegImg; c++) {
-    if (c >= nc) continue;
-    const size_t o = (size_t)(xa + c) * g.ldz + iz;
-#pragma unroll
-    for (int q = 0; q < 6; q++) a.G[q][o] += acc[c][q];
-  }
+  for (int q = 0; q < 6; q++) a.G[q][o] += acc[q];
 }
 
 constexpr int kPipeAdj = 2;
@@ -468,10 +403,11 @@ void launch_adj_vsweep(hipStream_t s, const AdjArgs &a)
   hipLaunchKernelGGL(k_adj_vsweep, tiles_grid(a.g, a.g.segLS), dim3(64), 0, s, a);
 }
 
-void launch_image_blk(hipStream_t s, const AdjArgs &a, const ImgBlock &b)
+void launch_image(hipStream_t s, const AdjArgs &a, const ImgBlock &b)
 {
-  if (b.hv[0][0]) hipLaunchKernelGGL(k_image_blk<true>, tiles_grid(a.g, kSegImg), dim3(64), 0, s, a, b);
-  else hipLaunchKernelGGL(k_image_blk<false>, tiles_grid(a.g, kSegImg), dim3(64), 0, s, a, b);
+  dim3 grid((a.g.izEnd - a.g.izStart + 255) / 256, a.g.ixEnd - a.g.ixStart);
+  if (b.e16[0][0]) hipLaunchKernelGGL(k_image<true>, grid, dim3(256), 0, s, a, b);
+  else hipLaunchKernelGGL(k_image<false>, grid, dim3(256), 0, s, a, b);
 }
 
 // ---------------------------------------------------------------- free surface

@@ -152,7 +152,31 @@ __device__ __forceinline__ void load_sops(SOps &p, const SweepArgs &a, size_t o,
 //   FORM_BOND   BT:836-873
 //   FORM_TENSOR TR:799-836 and the sigma <- R(tau) sweep TR:839-852, fused pointwise
 //   FORM_COORD  CR:804-845
-template <int FORM, int PD>
+// Store one column of the strain fields for the gradient (all lanes call it).
+template <int ESTORE>
+__device__ __forceinline__ void store_strain(const SweepArgs &a, size_t o, int c, int btz, bool zok, float e0,
+                                             float e1, float e2)
+{
+  const float e[3] = {e0, e1, e2};
+  if (ESTORE == 1) {
+    if (zok)
+#pragma unroll
+      for (int f = 0; f < 3; f++) a.e32[f][o] = e[f];
+    return;
+  }
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    float m = fabsf(e[f]);
+    for (int s = 32; s > 0; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+    int E = 0;
+    if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
+    E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
+    if (zok) a.e16[f][o] = (_Float16)(e[f] * ldexpf(1.0f, 14 - E));
+    if (threadIdx.x == 0) a.esc[f][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
+  }
+}
+
+template <int FORM, int PD, int ESTORE>
 __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx)
 {
   __shared__ float ring[2][kRing][LZ];
@@ -197,6 +221,7 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
       __syncthreads();
       const size_t o = (size_t)c * g.ldz + iz;
       float nxx = 0.0f, nzz = 0.0f, nxz = 0.0f;
+      float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
       if (zok) {
         const SOps &op = opq[0];
         float d1, d2;
@@ -206,6 +231,11 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
         rsg_bwd([&](int dx, int dz) { return ring[1][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, d1, d2);
         float vzdx = 0.5f * (d1 + d2) * invdx;
         float vzdz = 0.5f * (d1 - d2) * invdz;
+        if (ESTORE) {
+          e0 = vxdx;
+          e1 = vzdz;
+          e2 = vzdx + vxdz;
+        }
         const float k1 = op.k1, k2 = op.k2;
         if (FORM == FDMOD_FORM_BOND) {
           const float c15 = op.a, c35 = op.b;
@@ -262,6 +292,7 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
         st_out(a.szz + o, nzz);
         st_out(a.sxz + o, nxz);
       }
+      if (ESTORE) store_strain<ESTORE>(a, o, c, btz, zok, e0, e1, e2);
 #pragma unroll
       for (int i = 0; i + 1 < PD; i++) {
         colq[i] = colq[i + 1];
@@ -291,12 +322,12 @@ __global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
   vsweep_tile<COORD, PD>(a, btz, btx);
 }
 
-template <int FORM, int PD>
+template <int FORM, int PD, int ESTORE = 0>
 __global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
 {
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLS, btz, btx)) return;
-  ssweep_tile<FORM, PD>(a, btz, btx);
+  ssweep_tile<FORM, PD, ESTORE>(a, btz, btx);
 }
 
 // Batched grids: one launch covers the tiles of every shot.  start[s] is the first
@@ -339,7 +370,7 @@ __global__ __launch_bounds__(64) void k_ssweep_batch(const SweepArgs *__restrict
   if (!batch_tile(start, nshots, s, lt)) return;
   const SweepArgs a = shots[s];
   const int btx = lt / a.g.tilesZ;
-  ssweep_tile<FORM, PD>(a, lt - btx * a.g.tilesZ, btx);
+  ssweep_tile<FORM, PD, 0>(a, lt - btx * a.g.tilesZ, btx);
 }
 
 static dim3 sweep_grid(const Grid &g, int segL)
@@ -362,6 +393,8 @@ void launch_ssweep(hipStream_t s, int form, const SweepArgs &a)
   dim3 grid = sweep_grid(a.g, a.g.segLS);
   if (form == FDMOD_FORM_COORD) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_COORD, kPipeS>), grid, dim3(64), 0, s, a);
   else if (form == FDMOD_FORM_TENSOR) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_TENSOR, kPipeS>), grid, dim3(64), 0, s, a);
+  else if (a.estore == 1) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_BOND, kPipeS, 1>), grid, dim3(64), 0, s, a);
+  else if (a.estore == 2) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_BOND, kPipeS, 2>), grid, dim3(64), 0, s, a);
   else hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_BOND, kPipeS>), grid, dim3(64), 0, s, a);
 }
 

@@ -37,7 +37,7 @@ class GradOpts(ct.Structure):
     _fields_ = [("ncheckpoints", ct.c_int), ("storage", ct.c_int), ("history", ct.c_int)]
 
 
-STORAGE = {"f32": 0, "f16": 1}
+STORAGE = {"f32": 0, "f16": 1, "f16all": 2}
 
 SNAPSHOT_FN = ct.CFUNCTYPE(ct.c_int, ct.c_void_p, ct.c_int, ct.c_char_p, _fp, ct.c_int, ct.c_int)
 
