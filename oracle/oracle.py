@@ -177,18 +177,18 @@ class Oracle:
 
     def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, want_gc=False):
         """J = 1/2 sum (d - obs)^2 and dJ/d(vp, eps, delta, theta[deg]) as (4, nx, nz),
-        runs the forward from rest (Bond form).  want_gc also returns dJ/dc for the
-        six padded stiffness arrays, (6, Nx, Nz)."""
+        runs the forward from rest (all three forms).  want_gc also returns the
+        imaging sums of the per-point arrays the loop reads, (8, Nx, Nz)."""
         J = ct.c_double(0)
         g = np.zeros(4 * self.nx * self.nz)
-        gc = np.zeros(6 * self.Nx * self.Nz) if want_gc else None
+        gc = np.zeros(8 * self.Nx * self.Nz) if want_gc else None
         obs = [None if o is None else _f32(o).reshape(-1) for o in (obs_vx, obs_vz, obs_p)]
         rc = self.L["gradient"](self.c, *[None if o is None else _ptr(o, ct.c_float) for o in obs], ct.byref(J),
                                 _ptr(g, ct.c_double), None if gc is None else _ptr(gc, ct.c_double))
         if rc != 0:
             raise ValueError(self.L["last_error"](self.c).decode())
         g = g.reshape(4, self.nx, self.nz)
-        return (J.value, g, gc.reshape(6, self.Nx, self.Nz)) if want_gc else (J.value, g)
+        return (J.value, g, gc.reshape(8, self.Nx, self.Nz)) if want_gc else (J.value, g)
 
     def adjoint_source(self, y_vx=None, y_vz=None, y_p=None, ns=1):
         """F^T y: the adjoint of the source time functions, (ns, nt)."""

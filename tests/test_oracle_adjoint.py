@@ -1,4 +1,4 @@
-"""Pins of the FP64 oracle adjoint and gradient (Bond form).
+"""Pins of the FP64 oracle adjoint and gradient (all three rotation forms).
 
 The adjoint of the modelling operator is pinned by the dot-product test
 <F s, y> = <s, F^T y>; the gradient of J = 1/2 |d - obs|^2 by central finite
@@ -19,8 +19,11 @@ def _model():
     return synth.model_small(nz=NZ, nx=NX, seed=3)
 
 
-def _mk(model, fs=0, stype=1, seis=2, mode=1, mt=1):
-    o = Oracle("bond", NZ, NX, NT, 10.0, 10.0, DT, ealThick=10, freesurface=fs, fpeak=15.0, mode=mode, prec="f64")
+FORMS = ["bond", "tensor", "coord"]
+
+
+def _mk(model, fs=0, stype=1, seis=2, mode=1, mt=1, form="bond"):
+    o = Oracle(form, NZ, NX, NT, 10.0, 10.0, DT, ealThick=10, freesurface=fs, fpeak=15.0, mode=mode, prec="f64")
     o.set_model(model["vp"], model["vs"], model["rho"], model["eps"], model["delta"], model["theta"])
     o.set_source(stype, [30], [6], synth.ricker(15.0, DT, NT)[None])
     rx = np.arange(0, NX, 2)
@@ -28,11 +31,12 @@ def _mk(model, fs=0, stype=1, seis=2, mode=1, mt=1):
     return o
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("fs,stype,seis,mode,mt", [(0, 1, 2, 1, 1), (1, 1, 3, 1, 2), (0, 2, 3, 2, 1),
                                                    (1, 4, 1, 1, 1), (1, 7, 3, 1, 3), (0, 8, 2, 1, 1)])
-def test_dot_product(fs, stype, seis, mode, mt):
+def test_dot_product(fs, stype, seis, mode, mt, form):
     rng = np.random.default_rng(stype)
-    o = _mk(_model(), fs, stype, seis, mode, mt)
+    o = _mk(_model(), fs, stype, seis, mode, mt, form)
     o.run()
     use = {1: seis in (1, 3), 3: seis in (1, 3), 4: seis in (2, 3)}
     d = {c: o.traces(c) for c in (1, 3, 4)}
@@ -44,15 +48,15 @@ def test_dot_product(fs, stype, seis, mode, mt):
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
-def _setup(fs, seis=2):
+def _setup(fs, seis=2, form="bond"):
     m = _model()
     true = {k: v.copy() for k, v in m.items()}
     true["vp"] = (true["vp"] * 1.03).astype(np.float32)
     true["theta"] = (true["theta"] + 4).astype(np.float32)
-    ot = _mk(true, fs, seis=seis)
+    ot = _mk(true, fs, seis=seis, form=form)
     ot.run()
     obs = {c: ot.traces(c).astype(np.float32) for c in (1, 3, 4)}
-    o = _mk(m, fs, seis=seis)
+    o = _mk(m, fs, seis=seis, form=form)
     kw = dict(obs_p=obs[4]) if seis == 2 else dict(obs_vx=obs[1], obs_vz=obs[3], obs_p=obs[4])
     J, g = o.gradient(**kw)
     comps = (4,) if seis == 2 else (1, 3, 4)
@@ -65,9 +69,10 @@ def _setup(fs, seis=2):
     return m, J, g, Jof
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("fs,seis", [(0, 2), (1, 3)])
-def test_gradient_random_directions(fs, seis):
-    m, J, g, Jof = _setup(fs, seis)
+def test_gradient_random_directions(fs, seis, form):
+    m, J, g, Jof = _setup(fs, seis, form)
     assert abs(Jof(m) - J) <= 1e-12 * J
     rng = np.random.default_rng(10 + fs)
     for key, gi, sc in (("vp", 0, 0.5), ("eps", 1, 1e-4), ("delta", 2, 1e-4), ("theta", 3, 0.02)):

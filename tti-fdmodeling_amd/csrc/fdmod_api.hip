@@ -493,6 +493,18 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
     launch_source(c->stream, src);
   }
   launch_ssweep(c->stream, p.form, sa);
+  const bool snap_now = record && c->wsave > 0 && it % c->snap_sample == 0;
+  const bool rec_now = record && c->seistype > 0 && c->nr > 0 && it % c->mt == 0;
+  RecvArgs r{};
+  if (rec_now) {
+    r.nr = c->nr; r.ldz = c->g.ldz; r.gx = c->d_gx; r.gz = c->d_gz;
+    r.vx = sa.vx; r.vz = sa.vz; r.sxx = sa.sxx; r.szz = sa.szz;
+    r.out_vx = c->d_rvx; r.out_vz = c->d_rvz; r.out_p = c->d_rp;
+    r.isample = it / c->mt;
+    r.want_v = c->seistype == 1 || c->seistype == 3;
+    r.want_p = c->seistype == 2 || c->seistype == 3;
+    r.mt = c->mt;
+  }
   if (c->ns > 0 && c->stype == 1) {
     src.fa = sa.sxx;
     src.fb = sa.szz;
@@ -503,21 +515,11 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
     launch_source(c->stream, src);
   }
   if (!record) return FDMOD_OK;
-  if (c->wsave > 0 && it % c->snap_sample == 0) {
+  if (snap_now) {
     int rc = snapshot(c, it);
     if (rc) return rc;
   }
-  if (c->seistype > 0 && c->nr > 0 && it % c->mt == 0) {
-    RecvArgs r;
-    r.nr = c->nr; r.ldz = c->g.ldz; r.gx = c->d_gx; r.gz = c->d_gz;
-    r.vx = sa.vx; r.vz = sa.vz; r.sxx = sa.sxx; r.szz = sa.szz;
-    r.out_vx = c->d_rvx; r.out_vz = c->d_rvz; r.out_p = c->d_rp;
-    r.isample = it / c->mt;
-    r.want_v = c->seistype == 1 || c->seistype == 3;
-    r.want_p = c->seistype == 2 || c->seistype == 3;
-    r.mt = c->mt;
-    launch_receivers(c->stream, r);
-  }
+  if (rec_now) launch_receivers(c->stream, r);
   if (c->energy) {
     launch_energy_partial(c->stream, c->g, p.nx, p.nz, c->d, c->d_partial, c->eblocks);
     launch_energy_final(c->stream, c->d_partial, c->eblocks, p.dx, p.dz, c->d_E, c->d_Ek, c->d_Ep, it);

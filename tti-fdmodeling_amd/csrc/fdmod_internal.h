@@ -59,7 +59,8 @@ struct SweepArgs {
   int freesurface;
   // Strain output of the stress sweep for the gradient (Bond): (vxdx, vzdz, vzdx + vxdz)
   // at every updated point, as FP32 (estore 1) or as FP16 with one power-of-two scale
-  // per field, column and 64-row strip of the sweep (estore 2, scales [Nx][tilesZ]).
+  // per column and 64-row strip of the sweep, shared by the three fields (estore 2,
+  // scales [Nx][tilesZ] in esc[0]).
   int estore;
   float *e32[3];
   _Float16 *e16[3];

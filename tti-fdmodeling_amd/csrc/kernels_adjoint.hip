@@ -352,9 +352,10 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
     const double X = b.lam[j][0][o], Z = b.lam[j][1][o], Y = b.lam[j][2][o];
     double ex, ez, eg;
     if (F16) {
-      ex = (float)b.e16[j][0][o] * b.esc[j][0][so];
-      ez = (float)b.e16[j][1][o] * b.esc[j][1][so];
-      eg = (float)b.e16[j][2][o] * b.esc[j][2][so];
+      const float sc = b.esc[j][0][so];
+      ex = (float)b.e16[j][0][o] * sc;
+      ez = (float)b.e16[j][1][o] * sc;
+      eg = (float)b.e16[j][2][o] * sc;
     } else {
       ex = b.e32[j][0][o];
       ez = b.e32[j][1][o];

@@ -164,16 +164,17 @@ __device__ __forceinline__ void store_strain(const SweepArgs &a, size_t o, int c
       for (int f = 0; f < 3; f++) a.e32[f][o] = e[f];
     return;
   }
+  // one scale for the three strain components of the column block (same units)
+  float m = fmaxf(fabsf(e0), fmaxf(fabsf(e1), fabsf(e2)));
+  for (int s = 32; s > 0; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+  int E = 0;
+  if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
+  E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
+  const float up = ldexpf(1.0f, 14 - E);
+  if (zok)
 #pragma unroll
-  for (int f = 0; f < 3; f++) {
-    float m = fabsf(e[f]);
-    for (int s = 32; s > 0; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
-    int E = 0;
-    if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
-    E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
-    if (zok) a.e16[f][o] = (_Float16)(e[f] * ldexpf(1.0f, 14 - E));
-    if (threadIdx.x == 0) a.esc[f][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
-  }
+    for (int f = 0; f < 3; f++) a.e16[f][o] = (_Float16)(e[f] * up);
+  if (threadIdx.x == 0) a.esc[0][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
 }
 
 template <int FORM, int PD, int ESTORE>
