@@ -168,7 +168,7 @@ int fdmod_batch_reset(fdmod_batch *b);
 int fdmod_batch_get_traces(fdmod_batch *b, int shot, int component, float *out);
 
 /* ---- Adjoint and misfit gradient (no counterpart in the reference, which only runs
- * the forward loop; Bond form).  The adjoint is the exact transpose of the loop
+ * the forward loop; all three forms).  The adjoint is the exact transpose of the loop
  * iteration BT:651-954 as implemented by fdmod_run (sweeps, velocity/stress sources,
  * free-surface imaging, receiver sampling), with the absorbing layers (MEAL D, Beta,
  * coef1, coef2 of the last fdmod_set_model) held fixed.
@@ -203,7 +203,7 @@ typedef struct fdmod_grad_opts {
 int fdmod_gradient_opts(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, const float *obs_p,
                         const fdmod_grad_opts *opts, double *misfit, double *grad);
 
-/* Misfit gradient of every shot of the batch (Bond form), each as fdmod_gradient_opts
+/* Misfit gradient of every shot of the batch, each as fdmod_gradient_opts
  * would compute it, with the shots of a group advancing together on their own streams
  * (groups are as large as free device memory allows for the chosen storage).  obs_* are
  * arrays of nshots pointers (the array itself NULL = component absent).  With x0, shot s

@@ -1,6 +1,6 @@
 // libfdmod_hip adjoint entry points: the misfit gradient, the adjoint of the
 // modelling operator, the device dot-product test and the frozen-absorber model
-// update (Bond form).  Declarations are in include/fdmod_hip.h.
+// update, for the three rotation forms.  Declarations are in include/fdmod_hip.h.
 //
 // Gradient schedule:
 //   1. forward from rest, recording traces; with checkpoints every seg-th state
@@ -28,16 +28,22 @@ using namespace fdm;
 namespace {
 
 constexpr int kRedBlocks = 256;  // partial sums of the FP64 reductions
-constexpr int kNState = 5;       // vx, vz, sxx, szz, sxz
+constexpr int kMaxState = 8;     // vx, vz, sxx, szz, sxz (+ txx, tzz, txz for the tensor form)
+constexpr int kNG = 8;           // imaging sums
 
 size_t field_floats(const fdmod_ctx *c) { return (size_t)c->g.Nx * c->g.ldz; }
 
-const int kStateIds[kNState] = {FDMOD_A_VX, FDMOD_A_VZ, FDMOD_A_SXX, FDMOD_A_SZZ, FDMOD_A_SXZ};
+const int kStateIds[kMaxState] = {FDMOD_A_VX,  FDMOD_A_VZ,  FDMOD_A_SXX, FDMOD_A_SZZ,
+                                  FDMOD_A_SXZ, FDMOD_A_TXX, FDMOD_A_TZZ, FDMOD_A_TXZ};
+
+// Per-form counts: state (= adjoint) fields, imaging snapshots, history fields.
+int n_state(int form) { return form == FDMOD_FORM_TENSOR ? 8 : 5; }
+int n_snap(int form) { return form == FDMOD_FORM_BOND ? 3 : (form == FDMOD_FORM_TENSOR ? 6 : 5); }
+int n_hist(int form) { return form == FDMOD_FORM_BOND ? 3 : (form == FDMOD_FORM_TENSOR ? 6 : 8); }
 
 int check_ready(fdmod_ctx *c, const char *what)
 {
   if (!c->model_set) return fail(c, FDMOD_ESTATE, "%s before fdmod_set_model", what);
-  if (c->p.form != FDMOD_FORM_BOND) return fail(c, FDMOD_EINVAL, "%s: only the Bond form has an adjoint", what);
   if (c->seistype == 0 || c->nr == 0) return fail(c, FDMOD_ESTATE, "%s needs receivers", what);
   return FDMOD_OK;
 }
@@ -50,11 +56,11 @@ int workspace(fdmod_ctx *c, bool need_G)
     c->d_mu = nullptr; c->d_G = nullptr; c->d_snap = nullptr;
     c->d_mu_n = 0;
   }
-  if (!c->d_mu) HIPCALL(c, hipMalloc(&c->d_mu, kNState * n * sizeof(float)));
+  if (!c->d_mu) HIPCALL(c, hipMalloc(&c->d_mu, kMaxState * n * sizeof(float)));
   c->d_mu_n = n;
   if (!c->d_red) HIPCALL(c, hipMalloc(&c->d_red, (kRedBlocks + 8) * sizeof(double)));
-  if (need_G && !c->d_G) HIPCALL(c, hipMalloc(&c->d_G, 6 * n * sizeof(double)));
-  if (need_G && !c->d_snap) HIPCALL(c, hipMalloc(&c->d_snap, (size_t)kImgK * 3 * n * sizeof(float)));
+  if (need_G && !c->d_G) HIPCALL(c, hipMalloc(&c->d_G, kNG * n * sizeof(double)));
+  if (need_G && !c->d_snap) HIPCALL(c, hipMalloc(&c->d_snap, (size_t)kImgK * 6 * n * sizeof(float)));
   dfree(c->d_res);
   c->d_res = nullptr;
   HIPCALL(c, hipMalloc(&c->d_res, 3 * (size_t)c->nr * c->ntsample * sizeof(float)));
@@ -111,42 +117,37 @@ int store_io(fdmod_ctx *c, FieldStore &st, size_t slot, int nf, float *const *fi
   return FDMOD_OK;
 }
 
-// History slots hold the strain (vxdx, vzdz, vzdx + vxdz) of a step, 3 fields.
-constexpr int kNStrain = 3;
-
-// Point the stress sweep's strain output at history slot `slot`.
-SweepArgs with_strain(fdmod_ctx *c, const SweepArgs &sa, size_t slot, bool f16)
+// History slots hold the forward fields one step's imaging needs (n_hist fields,
+// two FP16 scale groups of scales_per_field each).
+SweepArgs with_hist(fdmod_ctx *c, const SweepArgs &sa, size_t slot, bool f16)
 {
   SweepArgs r = sa;
+  const int nh = n_hist(c->p.form);
   const size_t n = field_floats(c), ns = scales_per_field(c);
   r.estore = f16 ? 2 : 1;
-  for (int f = 0; f < kNStrain; f++) {
-    if (f16) {
-      r.e16[f] = (_Float16 *)c->hist.data + (slot * kNStrain + f) * n;
-      r.esc[f] = c->hist.scale + (slot * kNStrain + f) * ns;
-    } else {
-      r.e32[f] = (float *)c->hist.data + (slot * kNStrain + f) * n;
-    }
+  for (int f = 0; f < nh; f++) {
+    if (f16) r.h16[f] = (_Float16 *)c->hist.data + (slot * nh + f) * n;
+    else r.h32[f] = (float *)c->hist.data + (slot * nh + f) * n;
   }
+  if (f16)
+    for (int q = 0; q < 2; q++) r.hsc[q] = c->hist.scale + (slot * 2 + q) * ns;
   return r;
 }
 
 // Imaging block entry j: snapshot slot j and history slot `slot`.
 void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
 {
+  const int nh = n_hist(c->p.form);
   const size_t n = field_floats(c), ns = scales_per_field(c);
-  for (int q = 0; q < 3; q++) b.lam[j][q] = c->d_snap + ((size_t)j * 3 + q) * n;
-  for (int f = 0; f < kNStrain; f++) {
-    if (f16) {
-      b.e16[j][f] = (const _Float16 *)c->hist.data + (slot * kNStrain + f) * n;
-      b.esc[j][f] = c->hist.scale + (slot * kNStrain + f) * ns;
-      b.e32[j][f] = nullptr;
-    } else {
-      b.e32[j][f] = (const float *)c->hist.data + (slot * kNStrain + f) * n;
-      b.e16[j][f] = nullptr;
-      b.esc[j][f] = nullptr;
-    }
+  for (int q = 0; q < 6; q++) b.lam[j][q] = c->d_snap + ((size_t)j * 6 + q) * n;
+  for (int f = 0; f < 8; f++) {
+    b.h32[j][f] = nullptr;
+    b.h16[j][f] = nullptr;
+    if (f >= nh) continue;
+    if (f16) b.h16[j][f] = (const _Float16 *)c->hist.data + (slot * nh + f) * n;
+    else b.h32[j][f] = (const float *)c->hist.data + (slot * nh + f) * n;
   }
+  for (int q = 0; q < 2; q++) b.hsc[j][q] = f16 ? c->hist.scale + (slot * 2 + q) * ns : nullptr;
 }
 
 AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
@@ -160,12 +161,16 @@ AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
   }
   const size_t n = field_floats(c);
   float *const *d = c->d.a;
+  a.form = c->p.form;
   a.mvx = c->d_mu; a.mvz = c->d_mu + n; a.mxx = c->d_mu + 2 * n; a.mzz = c->d_mu + 3 * n; a.mxz = c->d_mu + 4 * n;
+  a.mtxx = c->d_mu + 5 * n; a.mtzz = c->d_mu + 6 * n; a.mtxz = c->d_mu + 7 * n;
   a.coef1 = d[FDMOD_A_COEF1]; a.coef2 = d[FDMOD_A_COEF2]; a.invrho = d[FDMOD_A_INVRHO];
   a.c11 = d[FDMOD_A_C11]; a.c13 = d[FDMOD_A_C13]; a.c15 = d[FDMOD_A_C15];
   a.c33 = d[FDMOD_A_C33]; a.c35 = d[FDMOD_A_C35]; a.c55 = d[FDMOD_A_C55];
-  a.snap[0] = a.snap[1] = a.snap[2] = nullptr;
-  for (int j = 0; j < 6; j++) a.G[j] = c->d_G ? c->d_G + j * n : nullptr;
+  a.sin2 = d[FDMOD_A_SIN2]; a.cos2 = d[FDMOD_A_COS2]; a.sin2t = d[FDMOD_A_SIN2T];
+  a.sn = d[FDMOD_A_SIN]; a.cs = d[FDMOD_A_COS]; a.snv = d[FDMOD_A_SINV]; a.csv = d[FDMOD_A_COSV];
+  for (int q = 0; q < 6; q++) a.snap[q] = nullptr;
+  for (int j = 0; j < kNG; j++) a.G[j] = c->d_G ? c->d_G + j * n : nullptr;
   return a;
 }
 
@@ -225,7 +230,11 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *src
   const bool rec = it % c->mt == 0;
   const size_t so = (size_t)(it / c->mt) * c->nr;
   if (rec && ds.p) launch_adj_inject(s, c->nr, c->g.ldz, c->d_gx, c->d_gz, ds.p + so, ds.p + so, a.mxx, a.mzz, 0.5f);
-  if (srcadj && c->ns > 0 && c->stype == 1) launch_adj_source(s, source_args(c, it), a.mxx, a.mzz, srcadj);
+  if (srcadj && c->ns > 0 && c->stype == 1) {
+    const bool tensor = p.form == FDMOD_FORM_TENSOR;  // TR:888-891: the source also feeds tau
+    launch_adj_source(s, source_args(c, it), a.mxx, a.mzz, tensor ? a.mtxx : nullptr, tensor ? a.mtzz : nullptr,
+                      srcadj);
+  }
   if (p.freesurface) launch_adj_fs(s, a);
   launch_adj_ssweep(s, a);
   if (rec && ds.vx)
@@ -233,27 +242,28 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *src
   if (srcadj && c->ns > 0 && c->stype != 1) {
     const float *ma = c->stype == 6 ? a.mvz : a.mvx;
     const float *mb = (c->stype == 5 || c->stype == 6) ? nullptr : a.mvz;
-    launch_adj_source(s, source_args(c, it), ma, mb, srcadj);
+    launch_adj_source(s, source_args(c, it), ma, mb, nullptr, nullptr, srcadj);
   }
   AdjArgs av = adj_args(c, true);
   if (snap_slot >= 0)
-    for (int q = 0; q < 3; q++) av.snap[q] = c->d_snap + ((size_t)snap_slot * 3 + q) * field_floats(c);
+    for (int q = 0; q < n_snap(p.form); q++) av.snap[q] = c->d_snap + ((size_t)snap_slot * 6 + q) * field_floats(c);
   launch_adj_vsweep(s, av);
 }
 
 int zero_adjoint(fdmod_ctx *c)
 {
   const size_t n = field_floats(c);
-  HIPCALL(c, hipMemsetAsync(c->d_mu, 0, kNState * n * sizeof(float), c->stream));
-  if (c->d_G) HIPCALL(c, hipMemsetAsync(c->d_G, 0, 6 * n * sizeof(double), c->stream));
+  HIPCALL(c, hipMemsetAsync(c->d_mu, 0, kMaxState * n * sizeof(float), c->stream));
+  if (c->d_G) HIPCALL(c, hipMemsetAsync(c->d_G, 0, kNG * n * sizeof(double), c->stream));
   return FDMOD_OK;
 }
 
 int state_io(fdmod_ctx *c, size_t slot, bool f16, bool save)
 {
-  float *f[kNState];
-  for (int i = 0; i < kNState; i++) f[i] = c->d.a[kStateIds[i]];
-  return store_io(c, c->ckpt, slot, kNState, f, f16, save);
+  const int ns = n_state(c->p.form);
+  float *f[kMaxState];
+  for (int i = 0; i < ns; i++) f[i] = c->d.a[kStateIds[i]];
+  return store_io(c, c->ckpt, slot, ns, f, f16, save);
 }
 
 
@@ -314,10 +324,10 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   if (r.history) {
     r.seg = p.nt;
     r.ncp = 1;
-    if ((rc = store_reserve(c, c->hist, p.nt, kNStrain, r.f16))) return rc;
+    if ((rc = store_reserve(c, c->hist, p.nt, n_hist(p.form), r.f16))) return rc;
   } else {
-    if ((rc = store_reserve(c, c->ckpt, r.ncp, kNState, r.ck16))) return rc;
-    if ((rc = store_reserve(c, c->hist, r.seg, kNStrain, r.f16))) return rc;
+    if ((rc = store_reserve(c, c->ckpt, r.ncp, n_state(p.form), r.ck16))) return rc;
+    if ((rc = store_reserve(c, c->hist, r.seg, n_hist(p.form), r.f16))) return rc;
   }
   if ((rc = fdmod_reset(c))) return rc;
   r.sa = sweep_args(c);
@@ -332,7 +342,7 @@ int grad_forward(GradRun &r, int k)
   int rc;
   if (!r.history && k % r.seg == 0 && (rc = state_io(c, k / r.seg, r.ck16, true))) return rc;
   const int it = p.mode == 1 ? k : p.nt - 1 - k;
-  if (r.history) return step(c, it, with_strain(c, r.sa, k, r.f16), true);
+  if (r.history) return step(c, it, with_hist(c, r.sa, k, r.f16), true);
   return step(c, it, r.sa, true);
 }
 
@@ -365,7 +375,7 @@ int grad_backward_next(GradRun &r)
   int rc;
   if (r.phase == 0) {
     if (r.k == k0 && (rc = state_io(c, r.sgi, r.ck16, false))) return rc;
-    if ((rc = step(c, p.mode == 1 ? r.k : p.nt - 1 - r.k, with_strain(c, r.sa, r.k - k0, r.f16), false)))
+    if ((rc = step(c, p.mode == 1 ? r.k : p.nt - 1 - r.k, with_hist(c, r.sa, r.k - k0, r.f16), false)))
       return rc;
     if (++r.k == k1) {
       r.phase = 1;
@@ -405,7 +415,7 @@ void swap_grad_mem(fdmod_ctx *c, GradMem &m)
 size_t held_bytes(const GradMem &m, size_t n)
 {
   size_t b = m.ckpt.cap_data + m.ckpt.cap_scale + m.hist.cap_data + m.hist.cap_scale;
-  if (m.mu) b += n * (kNState * 4 + 6 * 8 + kImgK * 3 * 4);
+  if (m.mu) b += n * (kMaxState * 4 + kNG * 8 + kImgK * 6 * 4);
   return b;
 }
 
@@ -414,14 +424,15 @@ size_t grad_bytes(const fdmod_ctx *c, const fdmod_grad_opts &o)
 {
   const size_t n = field_floats(c), nt = c->p.nt;
   const size_t esz = o.storage == FDMOD_STORE_F32 ? 4 : 2, cksz = o.storage == FDMOD_STORE_F16_ALL ? 2 : 4;
-  size_t b = n * (kNState * 4 + 6 * 8 + kImgK * 3 * 4);
+  const int form = c->p.form;
+  size_t b = n * (kMaxState * 4 + kNG * 8 + kImgK * 6 * 4);
   if (o.history) {
-    b += nt * kNStrain * n * esz;
+    b += nt * n_hist(form) * n * esz;
   } else {
     int ncp = o.ncheckpoints > 0 ? o.ncheckpoints : (int)ceil(sqrt((double)nt));
     ncp = std::min(std::max(ncp, 1), (int)nt);
     const size_t seg = (nt + ncp - 1) / ncp;
-    b += (nt + seg - 1) / seg * kNState * n * cksz + seg * kNStrain * n * esz;
+    b += (nt + seg - 1) / seg * n_state(form) * n * cksz + seg * n_hist(form) * n * esz;
   }
   return b + b / 16;  // block scales and slack
 }
@@ -432,9 +443,10 @@ int grad_finish(GradRun &r)
   fdmod_ctx *c = r.c;
   const fdmod_params &p = c->p;
   if (!c->d_grad) HIPCALL(c, hipMalloc(&c->d_grad, 4 * (size_t)p.nx * p.nz * sizeof(double)));
-  const int fs_row = p.freesurface ? c->g.e : -1;
-  launch_chain_bond(c->stream, c->g, p.nx, p.nz, fs_row, c->d, c->d_G, field_floats(c),
-                    (double)p.scalar / 180.0 * kPI, c->d_grad);
+  const int fs_row = p.freesurface && p.form == FDMOD_FORM_BOND ? c->g.e : -1;
+  if (p.form == FDMOD_FORM_COORD) launch_fold_trig(c->stream, c->g, c->d_G, field_floats(c));
+  launch_chain(c->stream, c->g, p.form, p.nx, p.nz, fs_row, c->d, c->d_G, field_floats(c),
+               (double)p.scalar / 180.0 * kPI, c->d_grad);
   HIPCALL(c, hipGetLastError());
   return FDMOD_OK;
 }

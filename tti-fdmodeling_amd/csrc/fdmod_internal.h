@@ -57,14 +57,19 @@ struct SweepArgs {
   const float *sin2, *cos2, *sin2t, *sn, *cs, *snv, *csv;
   const uint8_t *undamped;  // per tile: coef1 == 1 and coef2 == dt everywhere in the tile
   int freesurface;
-  // Strain output of the stress sweep for the gradient (Bond): (vxdx, vzdz, vzdx + vxdz)
-  // at every updated point, as FP32 (estore 1) or as FP16 with one power-of-two scale
-  // per column and 64-row strip of the sweep, shared by the three fields (estore 2,
-  // scales [Nx][tilesZ] in esc[0]).
+  // History output for the gradient, stored at every updated point as FP32 (estore 1)
+  // or as FP16 with one power-of-two scale per column, 64-row strip and group of
+  // fields (estore 2; scales [Nx][tilesZ] in hsc[group]):
+  //   Bond        stress sweep (vxdx, vzdz, vzdx + vxdz) -> h[0..2], group 0
+  //   tensor      stress sweep strain as for Bond -> h[0..2] (group 0) and the new tau
+  //               (before the stress source) -> h[3..5] (group 1)
+  //   coordinate  stress sweep (vxdx, vxdz, vzdx, vzdz) before rotation -> h[0..3]
+  //               (group 0); velocity sweep (-sxxdz + sxzdx, -sxzdz + szzdx,
+  //               sxxdx + sxzdz, sxzdx + szzdz) -> h[4..7] (group 1)
   int estore;
-  float *e32[3];
-  _Float16 *e16[3];
-  float *esc[3];
+  float *h32[8];
+  _Float16 *h16[8];
+  float *hsc[2];
 };
 
 // Source injection table entry: offset from the source cell and the two
@@ -96,38 +101,41 @@ struct RecvArgs {
   int mt;                   // receiver sampling (batched launches derive isample from it)
 };
 
-// Arguments of the adjoint sweeps (Bond form).  g is the grid of the launch: for
-// the velocity-sweep adjoint under a free surface its izStart is lowered by kN so the
+// Arguments of the adjoint sweeps.  g is the grid of the launch: for the
+// velocity-sweep adjoint under a free surface its izStart is lowered by kN so the
 // image rows receive their adjoint; izU is the first row of the update range.
 struct AdjArgs {
   Grid g;
-  int izU;
+  int izU, form;
   float *mvx, *mvz, *mxx, *mzz, *mxz;  // adjoint state
+  float *mtxx, *mtzz, *mtxz;           // tensor form: adjoint of tau
   const float *coef1, *coef2, *invrho;
   const float *c11, *c13, *c15, *c33, *c35, *c55;
-  float *snap[3];                      // V-adjoint: copy of the incoming mu_sigma (or null)
-  double *G[6];                        // imaging sums for c11, c13, c15, c33, c35, c55
+  const float *sin2, *cos2, *sin2t, *sn, *cs, *snv, *csv;
+  float *snap[6];                      // V-adjoint: imaging snapshots (or null)
+  double *G[8];                        // imaging sums (per-form meaning, kernels_adjoint.hip)
 };
 
-// One imaging launch over nk <= kImgK steps: the mu_sigma snapshots and the strain
-// E(v_{k+1}) the stress sweep stored for each step (FP32 e32, or FP16 e16 with the
-// per-(column, strip) scales esc).
+// One imaging launch over nk <= kImgK steps: the V-adjoint's snapshots of each step
+// (Bond 3, tensor 6, coordinate 5 fields) and the forward history of the step
+// (Bond 3, tensor 6, coordinate 8 fields; FP32 h32, or FP16 h16 with the per-(column,
+// strip) scales hsc of its two field groups).
 constexpr int kImgK = 8;
 struct ImgBlock {
   int nk;
-  const float *lam[kImgK][3];
-  const float *e32[kImgK][3];
-  const _Float16 *e16[kImgK][3];
-  const float *esc[kImgK][3];
+  const float *lam[kImgK][6];
+  const float *h32[kImgK][8];
+  const _Float16 *h16[kImgK][8];
+  const float *hsc[kImgK][2];
 };
 
 // FP16 packing of nf padded fields: dst = src * 2^(14-E) per (column, 64-row block),
 // inv = 2^(E-14) per block ([nx][ldz/64]); unpacking writes src back.
 struct Pack16 {
-  int nx, ldz, nf;
-  float *src[5];
-  _Float16 *dst[5];
-  float *inv[5];
+  int nx, ldz, nf;  // nf <= 8
+  float *src[8];
+  _Float16 *dst[8];
+  float *inv[8];
 };
 
 // ---- launchers (kernels_setup.hip) ----
@@ -167,12 +175,14 @@ void launch_image(hipStream_t s, const AdjArgs &a, const ImgBlock &b);
 void launch_adj_fs(hipStream_t s, const AdjArgs &a);
 void launch_adj_inject(hipStream_t s, int nr, int ldz, const int *gx, const int *gz, const float *r1,
                        const float *r2, float *f1, float *f2, float scale);
-void launch_adj_source(hipStream_t s, const SourceArgs &a, const float *ma, const float *mb, double *out);
+void launch_adj_source(hipStream_t s, const SourceArgs &a, const float *ma, const float *mb, const float *mc,
+                       const float *md, double *out);
 void launch_residual(hipStream_t s, const float *d, float *r, size_t n, double *partial, int nb, double *out);
 void launch_dot(hipStream_t s, const float *a, const float *bf, const double *bd, size_t n, double *partial, int nb,
                 double *out);
-void launch_chain_bond(hipStream_t s, const Grid &g, int nx, int nz, int fs_row, const DevArrays &d,
-                       const double *G, size_t gstride, double theta_scale, double *grad);
+void launch_chain(hipStream_t s, const Grid &g, int form, int nx, int nz, int fs_row, const DevArrays &d,
+                  const double *G, size_t gstride, double theta_scale, double *grad);
+void launch_fold_trig(hipStream_t s, const Grid &g, double *G, size_t gstride);
 void launch_rescale_frozen(hipStream_t s, const Grid &g, float *vp, float *vs, float *rho, const float *Beta);
 void launch_pack16(hipStream_t s, const Pack16 &p, bool pack);
 void launch_accum_grad(hipStream_t s, const double *src, int nx, int nz, int x0, int nx_full, double *dst);

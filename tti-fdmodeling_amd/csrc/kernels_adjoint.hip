@@ -1,5 +1,6 @@
-// Adjoint kernels of the modelling loop and of the L2 trace-misfit gradient
-// (Bond form).  No reference counterpart: the reference only runs the forward loop.
+// Adjoint kernels of the modelling loop and of the L2 trace-misfit gradient, for the
+// three rotation forms.  No reference counterpart: the reference only runs the
+// forward loop.
 //
 // One forward step is u_{k+1} = FS(S(V(u_k) + s^v_k)) + s^sigma_k and d_k = R u_{k+1}.
 // Going backwards through step k the adjoint state mu (same five fields) takes
@@ -24,54 +25,101 @@
 namespace fdm {
 
 // ---------------------------------------------------------------- S-sweep adjoint
-// Staged per row: mu_sigma (3), the six stiffness values and coef2.
+// The staged quantity M is the adjoint of the velocity gradients the stress sweep
+// reads at a stress point, computed pointwise from the adjoint stress and the
+// parameters (M vanishes outside U because the adjoint stress does there):
+//   Bond    M = coef2 C mu_sigma for (vxdx, vzdz, vzdx + vxdz)                 3 fields
+//   tensor  a = Q^T mu_sigma + mu_tau, e-adjoint = coef2 C_vti a, then the
+//           transpose of the strain rotation, for (vxdx, vzdz, vxdz + vzdx)    3 fields
+//   coord   VTI adjoint of the rotated gradients, rotated back, for
+//           (vxdx, vxdz, vzdx, vzdz)                                            4 fields
+constexpr int kRowMax = 14;
 struct MRow {
-  float s1, s2, s3, c11, c13, c15, c33, c35, c55, k2;
+  float v[kRowMax];
 };
 struct MCol {
   MRow lo, hi;
 };
 
+template <int FORM>
 __device__ __forceinline__ void load_mrow(MRow &r, const AdjArgs &a, size_t o)
 {
-  r.s1 = a.mxx[o]; r.s2 = a.mzz[o]; r.s3 = a.mxz[o];
-  r.c11 = a.c11[o]; r.c13 = a.c13[o]; r.c15 = a.c15[o];
-  r.c33 = a.c33[o]; r.c35 = a.c35[o]; r.c55 = a.c55[o];
-  r.k2 = a.coef2[o];
+  r.v[0] = a.mxx[o]; r.v[1] = a.mzz[o]; r.v[2] = a.mxz[o];
+  r.v[3] = a.c11[o]; r.v[4] = a.c13[o];
+  if (FORM == FDMOD_FORM_BOND) {
+    r.v[5] = a.c15[o]; r.v[6] = a.c33[o]; r.v[7] = a.c35[o]; r.v[8] = a.c55[o]; r.v[9] = a.coef2[o];
+  } else if (FORM == FDMOD_FORM_TENSOR) {
+    r.v[5] = a.c33[o]; r.v[6] = a.c55[o]; r.v[7] = a.coef2[o];
+    r.v[8] = a.mtxx[o]; r.v[9] = a.mtzz[o]; r.v[10] = a.mtxz[o];
+    r.v[11] = a.sin2[o]; r.v[12] = a.cos2[o]; r.v[13] = a.sin2t[o];
+  } else {
+    r.v[5] = a.c33[o]; r.v[6] = a.c55[o]; r.v[7] = a.coef2[o]; r.v[8] = a.sn[o]; r.v[9] = a.cs[o];
+  }
 }
 
-__device__ __forceinline__ void zero_mrow(MRow &r) { r = MRow{0, 0, 0, 0, 0, 0, 0, 0, 0, 0}; }
-
+template <int FORM>
 __device__ __forceinline__ void load_mcol(MCol &c, const AdjArgs &a, int gx, int rs, int lane, const Grid &g)
 {
   const size_t base = (size_t)gx * g.ldz;
   const int r0 = rs + lane, r1 = rs + TZ + lane;
-  if (r0 >= 0 && r0 < g.Nz) load_mrow(c.lo, a, base + r0);
-  else zero_mrow(c.lo);
-  if (lane < kHalo && r1 >= 0 && r1 < g.Nz) load_mrow(c.hi, a, base + r1);
-  else zero_mrow(c.hi);
+  c.lo = MRow{};
+  c.hi = MRow{};
+  if (r0 >= 0 && r0 < g.Nz) load_mrow<FORM>(c.lo, a, base + r0);
+  if (lane < kHalo && r1 >= 0 && r1 < g.Nz) load_mrow<FORM>(c.hi, a, base + r1);
 }
 
-// M = coef2 * C * mu_sigma: the adjoints of (vxdx, vzdz, vzdx + vxdz) at a stress point.
-__device__ __forceinline__ void m_of(const MRow &r, float &mxx, float &mzz, float &mg)
+template <int FORM>
+constexpr int m_fields()
 {
-  mxx = r.k2 * (r.c11 * r.s1 + r.c13 * r.s2 + r.c15 * r.s3);
-  mzz = r.k2 * (r.c13 * r.s1 + r.c33 * r.s2 + r.c35 * r.s3);
-  mg = r.k2 * (r.c15 * r.s1 + r.c35 * r.s2 + r.c55 * r.s3);
+  return FORM == FDMOD_FORM_COORD ? 4 : 3;
 }
 
+template <int FORM>
+__device__ __forceinline__ void m_of(const MRow &r, float (&m)[4])
+{
+  const float *v = r.v;
+  if (FORM == FDMOD_FORM_BOND) {
+    const float k2 = v[9];
+    m[0] = k2 * (v[3] * v[0] + v[4] * v[1] + v[5] * v[2]);
+    m[1] = k2 * (v[4] * v[0] + v[6] * v[1] + v[7] * v[2]);
+    m[2] = k2 * (v[5] * v[0] + v[7] * v[1] + v[8] * v[2]);
+    m[3] = 0.0f;
+  } else if (FORM == FDMOD_FORM_TENSOR) {
+    const float S2 = v[11], K2 = v[12], S2T = v[13], k2 = v[7];
+    const float a1 = K2 * v[0] + S2 * v[1] - 0.5f * S2T * v[2] + v[8];
+    const float a2 = S2 * v[0] + K2 * v[1] + 0.5f * S2T * v[2] + v[9];
+    const float a3 = S2T * v[0] - S2T * v[1] + (K2 - S2) * v[2] + v[10];
+    const float mexx = k2 * (v[3] * a1 + v[4] * a2);
+    const float mezz = k2 * (v[4] * a1 + v[5] * a2);
+    const float mexz = k2 * v[6] * a3;
+    m[0] = K2 * mexx + S2 * mezz + S2T * mexz;
+    m[1] = S2 * mexx + K2 * mezz - S2T * mexz;
+    m[2] = -0.5f * S2T * mexx + 0.5f * S2T * mezz + (K2 - S2) * mexz;
+    m[3] = 0.0f;
+  } else {
+    const float k2 = v[7], sn = v[8], cs = v[9];
+    const float rxdx = k2 * (v[3] * v[0] + v[4] * v[1]);
+    const float rzdz = k2 * (v[4] * v[0] + v[5] * v[1]);
+    const float rg = k2 * v[6] * v[2];
+    m[0] = cs * rxdx + sn * rg;   // vxdx
+    m[1] = -sn * rxdx + cs * rg;  // vxdz
+    m[2] = cs * rg + sn * rzdz;   // vzdx
+    m[3] = -sn * rg + cs * rzdz;  // vzdz
+  }
+}
+
+template <int FORM>
 __device__ __forceinline__ void store_mcol(float (*ring)[kRing][LZ], int slot, const MCol &c, int lane)
 {
-  float x, z, y;
-  m_of(c.lo, x, z, y);
-  ring[0][slot][lane] = x;
-  ring[1][slot][lane] = z;
-  ring[2][slot][lane] = y;
+  constexpr int NM = m_fields<FORM>();
+  float m[4];
+  m_of<FORM>(c.lo, m);
+#pragma unroll
+  for (int f = 0; f < NM; f++) ring[f][slot][lane] = m[f];
   if (lane < kHalo) {
-    m_of(c.hi, x, z, y);
-    ring[0][slot][TZ + lane] = x;
-    ring[1][slot][TZ + lane] = z;
-    ring[2][slot][TZ + lane] = y;
+    m_of<FORM>(c.hi, m);
+#pragma unroll
+    for (int f = 0; f < NM; f++) ring[f][slot][TZ + lane] = m[f];
   }
 }
 
@@ -79,11 +127,14 @@ struct VAdjOps {
   float mvx, mvz, k1;
 };
 
-// mu_v(p) <- coef1(p) mu_v(p) - (Dx M_xx + Dz M_g, Dx M_g + Dz M_zz)(p), p in U.
-template <int PD>
+// mu_v(p) <- coef1(p) mu_v(p) - (adjoint of the gradients of v read by the stress
+// sweep), p in U.  Bond/tensor: (Dx M0 + Dz M2, Dx M2 + Dz M1); coordinate:
+// (Dx M0 + Dz M1, Dx M2 + Dz M3).
+template <int FORM, int PD>
 __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
 {
-  __shared__ float ring[3][kRing][LZ];
+  constexpr int NM = m_fields<FORM>();
+  __shared__ float ring[NM][kRing][LZ];
   const Grid &g = a.g;
   const int segL = g.segLV;
   const int lane = threadIdx.x;
@@ -93,14 +144,14 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   const int iz = z0 + lane;
   const bool zok = iz < g.izEnd;
   const int rs = z0 - 4;
-  // prologue in groups of three columns (a staged row is 10 floats)
+  // prologue in groups of three columns (a staged row is up to 14 floats)
 #pragma unroll
   for (int m0 = 0; m0 < 9; m0 += 3) {
     MCol c[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) load_mcol(c[m], a, xa - 4 + m0 + m, rs, lane, g);
+    for (int m = 0; m < 3; m++) load_mcol<FORM>(c[m], a, xa - 4 + m0 + m, rs, lane, g);
 #pragma unroll
-    for (int m = 0; m < 3; m++) store_mcol(ring, m0 + m, c[m], lane);
+    for (int m = 0; m < 3; m++) store_mcol<FORM>(ring, m0 + m, c[m], lane);
   }
   MCol colq[PD];
   VAdjOps opq[PD];
@@ -112,7 +163,7 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   };
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    if (i <= nc - 1) load_mcol(colq[i], a, xa + 5 + i, rs, lane, g);
+    if (i <= nc - 1) load_mcol<FORM>(colq[i], a, xa + 5 + i, rs, lane, g);
     if (zok && i < nc) load_ops(opq[i], xa + i);
   }
   const float invdx = g.invdx, invdz = g.invdz;
@@ -124,23 +175,31 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
       const int c = xa + k;
       MCol cn;
       VAdjOps on;
-      if (k <= nc - PD - 1) load_mcol(cn, a, c + 5 + PD, rs, lane, g);
+      if (k <= nc - PD - 1) load_mcol<FORM>(cn, a, c + 5 + PD, rs, lane, g);
       if (zok && k + PD < nc) load_ops(on, c + PD);
-      store_mcol(ring, (u + 9) % kRing, colq[0], lane);
+      store_mcol<FORM>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
       if (zok) {
         float d1, d2;
         rsg_fwd([&](int dx, int dz) { return ring[0][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
-        const float mxxdx = 0.5f * (d1 + d2) * invdx;
+        const float m0dx = 0.5f * (d1 + d2) * invdx;
         rsg_fwd([&](int dx, int dz) { return ring[1][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
-        const float mzzdz = 0.5f * (d1 - d2) * invdz;
+        const float m1dz = 0.5f * (d1 - d2) * invdz;
         rsg_fwd([&](int dx, int dz) { return ring[2][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
-        const float mgdx = 0.5f * (d1 + d2) * invdx;
-        const float mgdz = 0.5f * (d1 - d2) * invdz;
+        const float m2dx = 0.5f * (d1 + d2) * invdx;
+        const float m2dz = 0.5f * (d1 - d2) * invdz;
         const size_t o = (size_t)c * g.ldz + iz;
         const VAdjOps &op = opq[0];
-        a.mvx[o] = op.k1 * op.mvx - (mxxdx + mgdz);
-        a.mvz[o] = op.k1 * op.mvz - (mgdx + mzzdz);
+        if (FORM != FDMOD_FORM_COORD) {
+          a.mvx[o] = op.k1 * op.mvx - (m0dx + m2dz);
+          a.mvz[o] = op.k1 * op.mvz - (m2dx + m1dz);
+        } else {
+          rsg_fwd([&](int dx, int dz) { return ring[NM - 1][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1,
+                  d2);
+          const float m3dz = 0.5f * (d1 - d2) * invdz;
+          a.mvx[o] = op.k1 * op.mvx - (m0dx + m1dz);
+          a.mvz[o] = op.k1 * op.mvz - (m2dx + m3dz);
+        }
       }
 #pragma unroll
       for (int i = 0; i + 1 < PD; i++) {
@@ -154,56 +213,94 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
 }
 
 // ---------------------------------------------------------------- V-sweep adjoint
-// Staged per row: mu_v (2), coef2, invrho.
+// Staged per row: mu_v (2), coef2, invrho and, for the coordinate form, the averaged
+// trig (cosv, sinv).  The staged fields N are the adjoints of the stress gradients
+// the velocity sweep reads, with a = coef2 invrho mu_vx, b = coef2 invrho mu_vz:
+//   Bond, tensor  (a, b)                                                  2 fields
+//   coordinate    (cv a, -sv a, sv a + cv b, cv a - sv b, sv b, cv b) for
+//                 (sxxdx, sxxdz, sxzdx, sxzdz, szzdx, szzdz)              6 fields
 struct NRow {
-  float vx, vz, k2, ir;
+  float vx, vz, k2, ir, cv, sv;
 };
 struct NCol {
   NRow lo, hi;
 };
 
+template <int FORM>
+__device__ __forceinline__ void load_nrow(NRow &r, const AdjArgs &a, size_t o)
+{
+  r = NRow{a.mvx[o], a.mvz[o], a.coef2[o], a.invrho[o], 0.0f, 0.0f};
+  if (FORM == FDMOD_FORM_COORD) {
+    r.cv = a.csv[o];
+    r.sv = a.snv[o];
+  }
+}
+
+template <int FORM>
 __device__ __forceinline__ void load_ncol(NCol &c, const AdjArgs &a, int gx, int rs, int lane, const Grid &g)
 {
   const size_t base = (size_t)gx * g.ldz;
   const int r0 = rs + lane, r1 = rs + TZ + lane;
-  if (r0 >= 0 && r0 < g.Nz) {
-    const size_t o = base + r0;
-    c.lo = NRow{a.mvx[o], a.mvz[o], a.coef2[o], a.invrho[o]};
-  } else {
-    c.lo = NRow{0, 0, 0, 0};
-  }
-  if (lane < kHalo && r1 >= 0 && r1 < g.Nz) {
-    const size_t o = base + r1;
-    c.hi = NRow{a.mvx[o], a.mvz[o], a.coef2[o], a.invrho[o]};
-  } else {
-    c.hi = NRow{0, 0, 0, 0};
-  }
+  c.lo = NRow{0, 0, 0, 0, 0, 0};
+  c.hi = NRow{0, 0, 0, 0, 0, 0};
+  if (r0 >= 0 && r0 < g.Nz) load_nrow<FORM>(c.lo, a, base + r0);
+  if (lane < kHalo && r1 >= 0 && r1 < g.Nz) load_nrow<FORM>(c.hi, a, base + r1);
 }
 
+template <int FORM>
+constexpr int n_fields()
+{
+  return FORM == FDMOD_FORM_COORD ? 6 : 2;
+}
+
+template <int FORM>
+__device__ __forceinline__ void n_of(const NRow &r, float (&n)[6])
+{
+  const float kr = r.k2 * r.ir;
+  const float av = kr * r.vx, bv = kr * r.vz;
+  if (FORM != FDMOD_FORM_COORD) {
+    n[0] = av;
+    n[1] = bv;
+    return;
+  }
+  n[0] = r.cv * av;
+  n[1] = -r.sv * av;
+  n[2] = r.sv * av + r.cv * bv;
+  n[3] = r.cv * av - r.sv * bv;
+  n[4] = r.sv * bv;
+  n[5] = r.cv * bv;
+}
+
+template <int FORM>
 __device__ __forceinline__ void store_ncol(float (*ring)[kRing][LZ], int slot, const NCol &c, int lane)
 {
-  float kr = c.lo.k2 * c.lo.ir;
-  ring[0][slot][lane] = kr * c.lo.vx;
-  ring[1][slot][lane] = kr * c.lo.vz;
+  constexpr int NN = n_fields<FORM>();
+  float n[6];
+  n_of<FORM>(c.lo, n);
+#pragma unroll
+  for (int f = 0; f < NN; f++) ring[f][slot][lane] = n[f];
   if (lane < kHalo) {
-    kr = c.hi.k2 * c.hi.ir;
-    ring[0][slot][TZ + lane] = kr * c.hi.vx;
-    ring[1][slot][TZ + lane] = kr * c.hi.vz;
+    n_of<FORM>(c.hi, n);
+#pragma unroll
+    for (int f = 0; f < NN; f++) ring[f][slot][TZ + lane] = n[f];
   }
 }
 
 struct SAdjOps {
-  float s1, s2, s3, k1;
+  float s1, s2, s3, k1, t1, t2, t3, S2, K2, S2T, na, nb;
 };
 
-// mu_sigma(q) <- k1(q) mu_sigma(q) - (Dx Nx, Dz Nz, Dz Nx + Dx Nz)(q), with k1 = coef1
-// inside U and 1 on the image rows above a free surface (rows [g.izStart, a.izU)).
-// The incoming mu_sigma is the one the imaging of this step uses; with a.snap set it
-// is also written there (points of U) for the blocked imaging kernel.
-template <int PD>
+// mu_sigma(q) <- k1(q) mu_sigma(q) - (stress-gradient adjoints)(q), with k1 = coef1 in U
+// and 1 on the image rows above a free surface (rows [g.izStart, a.izU)).  Tensor form:
+// sigma is overwritten on U, so there mu_sigma <- -(...), and mu_tau <- coef1 a with
+// a = Q^T mu_sigma + mu_tau.  With a.snap set, the incoming values the imaging of this
+// step needs are copied there (points of U): mu_sigma, then a (tensor) or (a, b)
+// (coordinate).
+template <int FORM, int PD>
 __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
 {
-  __shared__ float ring[2][kRing][LZ];
+  constexpr int NN = n_fields<FORM>();
+  __shared__ float ring[NN][kRing][LZ];
   const Grid &g = a.g;
   const int segL = g.segLS;
   const int lane = threadIdx.x;
@@ -214,25 +311,40 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
   const bool zok = iz < g.izEnd;
   const bool inU = iz >= a.izU;
   const int rs = z0 - 5;
-  {
-    NCol c[9];
 #pragma unroll
-    for (int m = 0; m < 9; m++) load_ncol(c[m], a, xa - 5 + m, rs, lane, g);
+  for (int m0 = 0; m0 < 9; m0 += 3) {
+    NCol c[3];
 #pragma unroll
-    for (int m = 0; m < 9; m++) store_ncol(ring, m, c[m], lane);
+    for (int m = 0; m < 3; m++) load_ncol<FORM>(c[m], a, xa - 5 + m0 + m, rs, lane, g);
+#pragma unroll
+    for (int m = 0; m < 3; m++) store_ncol<FORM>(ring, m0 + m, c[m], lane);
   }
   NCol colq[PD];
   SAdjOps opq[PD];
   auto load_ops = [&](SAdjOps &p, int col) {
     const size_t o = (size_t)col * g.ldz + iz;
+    p = SAdjOps{};
     p.s1 = a.mxx[o];
     p.s2 = a.mzz[o];
     p.s3 = a.mxz[o];
     p.k1 = inU ? a.coef1[o] : 1.0f;
+    if (FORM == FDMOD_FORM_TENSOR) {
+      p.t1 = a.mtxx[o];
+      p.t2 = a.mtzz[o];
+      p.t3 = a.mtxz[o];
+      p.S2 = inU ? a.sin2[o] : 0.0f;
+      p.K2 = inU ? a.cos2[o] : 0.0f;
+      p.S2T = inU ? a.sin2t[o] : 0.0f;
+    }
+    if (FORM == FDMOD_FORM_COORD && a.snap[0] && inU) {
+      const float kr = a.coef2[o] * a.invrho[o];
+      p.na = kr * a.mvx[o];
+      p.nb = kr * a.mvz[o];
+    }
   };
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    if (i <= nc - 1) load_ncol(colq[i], a, xa + 4 + i, rs, lane, g);
+    if (i <= nc - 1) load_ncol<FORM>(colq[i], a, xa + 4 + i, rs, lane, g);
     if (zok && i < nc) load_ops(opq[i], xa + i);
   }
   const float invdx = g.invdx, invdz = g.invdz;
@@ -244,23 +356,56 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
       const int c = xa + k;
       NCol cn;
       SAdjOps on;
-      if (k <= nc - PD - 1) load_ncol(cn, a, c + 4 + PD, rs, lane, g);
+      if (k <= nc - PD - 1) load_ncol<FORM>(cn, a, c + 4 + PD, rs, lane, g);
       if (zok && k + PD < nc) load_ops(on, c + PD);
-      store_ncol(ring, (u + 9) % kRing, colq[0], lane);
+      store_ncol<FORM>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
       if (zok) {
-        float d1, d2;
-        rsg_bwd([&](int dx, int dz) { return ring[0][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, d1, d2);
-        const float nxdx = 0.5f * (d1 + d2) * invdx;
-        const float nxdz = 0.5f * (d1 - d2) * invdz;
-        rsg_bwd([&](int dx, int dz) { return ring[1][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, d1, d2);
-        const float nzdx = 0.5f * (d1 + d2) * invdx;
-        const float nzdz = 0.5f * (d1 - d2) * invdz;
+        // d1, d2 of staged field f; then x- and z-derivative forms
+        float dd1[NN], dd2[NN];
+#pragma unroll
+        for (int f = 0; f < NN; f++)
+          rsg_bwd([&](int dx, int dz) { return ring[f][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, dd1[f],
+                  dd2[f]);
         const size_t o = (size_t)c * g.ldz + iz;
         const SAdjOps &op = opq[0];
-        a.mxx[o] = op.k1 * op.s1 - nxdx;
-        a.mzz[o] = op.k1 * op.s2 - nzdz;
-        a.mxz[o] = op.k1 * op.s3 - (nxdz + nzdx);
+        float gxx, gzz, gxz;  // adjoint contributions to subtract
+        if (FORM != FDMOD_FORM_COORD) {
+          const float nxdx = 0.5f * (dd1[0] + dd2[0]) * invdx, nxdz = 0.5f * (dd1[0] - dd2[0]) * invdz;
+          const float nzdx = 0.5f * (dd1[1] + dd2[1]) * invdx, nzdz = 0.5f * (dd1[1] - dd2[1]) * invdz;
+          gxx = nxdx;
+          gzz = nzdz;
+          gxz = nxdz + nzdx;
+        } else {
+          // sxxdz carries the reference's invdx (CR:633)
+          gxx = 0.5f * (dd1[0] + dd2[0]) * invdx + 0.5f * (dd1[1] - dd2[1]) * invdx;
+          gxz = 0.5f * (dd1[2] + dd2[2]) * invdx + 0.5f * (dd1[3] - dd2[3]) * invdz;
+          gzz = 0.5f * (dd1[4] + dd2[4]) * invdx + 0.5f * (dd1[5] - dd2[5]) * invdz;
+        }
+        if (FORM == FDMOD_FORM_TENSOR && inU) {
+          const float a1 = op.K2 * op.s1 + op.S2 * op.s2 - 0.5f * op.S2T * op.s3 + op.t1;
+          const float a2 = op.S2 * op.s1 + op.K2 * op.s2 + 0.5f * op.S2T * op.s3 + op.t2;
+          const float a3 = op.S2T * op.s1 - op.S2T * op.s2 + (op.K2 - op.S2) * op.s3 + op.t3;
+          a.mtxx[o] = op.k1 * a1;
+          a.mtzz[o] = op.k1 * a2;
+          a.mtxz[o] = op.k1 * a3;
+          a.mxx[o] = -gxx;
+          a.mzz[o] = -gzz;
+          a.mxz[o] = -gxz;
+          if (a.snap[0]) {
+            a.snap[3][o] = a1;
+            a.snap[4][o] = a2;
+            a.snap[5][o] = a3;
+          }
+        } else {
+          a.mxx[o] = op.k1 * op.s1 - gxx;
+          a.mzz[o] = op.k1 * op.s2 - gzz;
+          a.mxz[o] = op.k1 * op.s3 - gxz;
+          if (FORM == FDMOD_FORM_COORD && a.snap[0] && inU) {
+            a.snap[3][o] = op.na;
+            a.snap[4][o] = op.nb;
+          }
+        }
         if (a.snap[0] && inU) {  // mu_sigma as imaged at this step (after FS^T)
           a.snap[0][o] = op.s1;
           a.snap[1][o] = op.s2;
@@ -329,14 +474,17 @@ void launch_pack16(hipStream_t s, const Pack16 &p, bool pack)
 }
 
 // ---------------------------------------------------------------- imaging
-// G_j(q) += sum over a block of nk steps of (mu_sigma (x) E(v_{k+1}))_j, q in U, for
-// the six Bond stiffness arrays in the order c11, c13, c15, c33, c35, c55, where
-// E = (vxdx, vzdz, vzdx + vxdz) is the strain the forward stress sweep computed at q
-// and stored, and mu_sigma the snapshot the V-adjoint wrote.  Pointwise: one thread
-// per point keeps the six sums of the block in FP64 registers, so the FP64 arrays are
-// read and written once per block of steps.  coef2 is time-independent and
-// multiplies the sums in the chain-rule kernel.
-template <bool F16>
+// Per point of U, sum over a block of nk steps the products of the adjoint snapshots
+// (written by the V-adjoint) and the stored forward history (written by the forward
+// sweeps), into the FP64 imaging sums G.  One thread per point keeps the block's sums
+// in FP64 registers, so G is read and written once per block of steps.
+//   Bond    G = c11, c13, c15, c33, c35, c55 sums of mu_sigma (x) E, E = (vxdx, vzdz,
+//           vzdx + vxdz); coef2 multiplies them in the chain-rule kernel.
+//   tensor  G = c11, c13, c33, c55, sin^2, cos^2, sin 2theta (TR:826-852), from the
+//           snapshots mu_sigma and a = Q^T mu_sigma + mu_tau, the strain and tau_{k+1}.
+//   coord   G = c11, c13, c33, c55, sin, cos at stress points (CR:821-843) and sinv,
+//           cosv at velocity points (CR:663-670).
+template <int FORM, bool F16>
 __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
 {
   const Grid &g = a.g;
@@ -345,47 +493,98 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
   if (iz >= g.izEnd) return;
   const size_t o = (size_t)ix * g.ldz + iz;
   const size_t so = (size_t)ix * g.tilesZ + (iz - g.izStart) / TZ;
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  constexpr int NH = FORM == FDMOD_FORM_BOND ? 3 : (FORM == FDMOD_FORM_TENSOR ? 6 : 8);
+  constexpr int NL = FORM == FDMOD_FORM_BOND ? 3 : (FORM == FDMOD_FORM_TENSOR ? 6 : 5);
+  constexpr int NG = FORM == FDMOD_FORM_BOND ? 6 : (FORM == FDMOD_FORM_TENSOR ? 7 : 8);
+  double acc[NG];
+#pragma unroll
+  for (int q = 0; q < NG; q++) acc[q] = 0.0;
+  float P[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // time-independent parameters of the point
+  if (FORM == FDMOD_FORM_TENSOR) {
+    P[0] = a.c11[o]; P[1] = a.c13[o]; P[2] = a.c33[o]; P[3] = a.c55[o]; P[4] = a.coef2[o];
+    P[5] = a.sin2[o]; P[6] = a.cos2[o]; P[7] = a.sin2t[o];
+  } else if (FORM == FDMOD_FORM_COORD) {
+    P[0] = a.c11[o]; P[1] = a.c13[o]; P[2] = a.c33[o]; P[3] = a.c55[o]; P[4] = a.coef2[o];
+    P[5] = a.sn[o]; P[6] = a.cs[o];
+  }
 #pragma unroll
   for (int j = 0; j < kImgK; j++) {
     if (j >= b.nk) break;
-    const double X = b.lam[j][0][o], Z = b.lam[j][1][o], Y = b.lam[j][2][o];
-    double ex, ez, eg;
+    double L[NL], H[NH];
+#pragma unroll
+    for (int q = 0; q < NL; q++) L[q] = b.lam[j][q][o];
     if (F16) {
-      const float sc = b.esc[j][0][so];
-      ex = (float)b.e16[j][0][o] * sc;
-      ez = (float)b.e16[j][1][o] * sc;
-      eg = (float)b.e16[j][2][o] * sc;
+      const float s0 = b.hsc[j][0][so], s1 = NH > 4 ? b.hsc[j][1][so] : 0.0f;
+      const int n0 = FORM == FDMOD_FORM_COORD ? 4 : 3;
+#pragma unroll
+      for (int q = 0; q < NH; q++) H[q] = (float)b.h16[j][q][o] * (q < n0 ? s0 : s1);
     } else {
-      ex = b.e32[j][0][o];
-      ez = b.e32[j][1][o];
-      eg = b.e32[j][2][o];
+#pragma unroll
+      for (int q = 0; q < NH; q++) H[q] = b.h32[j][q][o];
     }
-    acc[0] += X * ex;
-    acc[1] += X * ez + Z * ex;
-    acc[2] += X * eg + Y * ex;
-    acc[3] += Z * ez;
-    acc[4] += Z * eg + Y * ez;
-    acc[5] += Y * eg;
+    if (FORM == FDMOD_FORM_BOND) {
+      const double X = L[0], Z = L[1], Y = L[2], ex = H[0], ez = H[1], eg = H[2];
+      acc[0] += X * ex;
+      acc[1] += X * ez + Z * ex;
+      acc[2] += X * eg + Y * ex;
+      acc[3] += Z * ez;
+      acc[4] += Z * eg + Y * ez;
+      acc[5] += Y * eg;
+    } else if (FORM == FDMOD_FORM_TENSOR) {
+      const double lx = L[0], lz = L[1], lc = L[2], a1 = L[3], a2 = L[4], a3 = L[5];
+      const double d = H[0], z = H[1], gg = H[2], t1 = H[3], t2 = H[4], t3 = H[5];
+      const double k2 = P[4], S2 = P[5], K2 = P[6], S2T = P[7];
+      const double exx = K2 * d + S2 * z - 0.5 * S2T * gg, ezz = S2 * d + K2 * z + 0.5 * S2T * gg;
+      const double exz = S2T * (d - z) + (K2 - S2) * gg;
+      const double mexx = k2 * ((double)P[0] * a1 + (double)P[1] * a2);
+      const double mezz = k2 * ((double)P[1] * a1 + (double)P[2] * a2);
+      const double mexz = k2 * (double)P[3] * a3;
+      acc[0] += k2 * a1 * exx;
+      acc[1] += k2 * (a1 * ezz + a2 * exx);
+      acc[2] += k2 * a2 * ezz;
+      acc[3] += k2 * a3 * exz;
+      acc[4] += mexx * z + mezz * d - mexz * gg + lx * t2 + lz * t1 - lc * t3;
+      acc[5] += mexx * d + mezz * z + mexz * gg + lx * t1 + lz * t2 + lc * t3;
+      acc[6] += -0.5 * mexx * gg + 0.5 * mezz * gg + mexz * (d - z) + lx * t3 - lz * t3 - 0.5 * lc * (t1 - t2);
+    } else {
+      const double lx = L[0], lz = L[1], lc = L[2];
+      const double X = H[0], XZ = H[1], ZX = H[2], Z = H[3];
+      const double k2 = P[4], sn = P[5], cs = P[6];
+      const double pxdx = cs * X - sn * XZ, pxdz = sn * X + cs * XZ;
+      const double pzdx = cs * ZX - sn * Z, pzdz = sn * ZX + cs * Z;
+      const double rxdx = k2 * ((double)P[0] * lx + (double)P[1] * lz);
+      const double rzdz = k2 * ((double)P[1] * lx + (double)P[2] * lz);
+      const double rg = k2 * (double)P[3] * lc;
+      acc[0] += k2 * lx * pxdx;
+      acc[1] += k2 * (lx * pzdz + lz * pxdx);
+      acc[2] += k2 * lz * pzdz;
+      acc[3] += k2 * lc * (pzdx + pxdz);
+      acc[4] += rxdx * (-XZ) + rg * X + rg * (-Z) + rzdz * ZX;
+      acc[5] += rxdx * X + rg * XZ + rg * ZX + rzdz * Z;
+      acc[6] += L[3] * H[4] + L[4] * H[5];  // sinv: a (-sxxdz + sxzdx) + b (-sxzdz + szzdx)
+      acc[7] += L[3] * H[6] + L[4] * H[7];  // cosv: a (sxxdx + sxzdz) + b (sxzdx + szzdz)
+    }
   }
 #pragma unroll
-  for (int q = 0; q < 6; q++) a.G[q][o] += acc[q];
+  for (int q = 0; q < NG; q++) a.G[q][o] += acc[q];
 }
 
 constexpr int kPipeAdj = 2;
 
+template <int FORM>
 __global__ __launch_bounds__(64) void k_adj_ssweep(AdjArgs a)
 {
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLV, btz, btx)) return;
-  adj_s_tile<kPipeAdj>(a, btz, btx);
+  adj_s_tile<FORM, kPipeAdj>(a, btz, btx);
 }
 
+template <int FORM>
 __global__ __launch_bounds__(64) void k_adj_vsweep(AdjArgs a)
 {
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLS, btz, btx)) return;
-  adj_v_tile<kPipeAdj>(a, btz, btx);
+  adj_v_tile<FORM, kPipeAdj>(a, btz, btx);
 }
 
 static dim3 tiles_grid(const Grid &g, int segL)
@@ -394,21 +593,37 @@ static dim3 tiles_grid(const Grid &g, int segL)
   return dim3((n + kXCD - 1) / kXCD * kXCD);
 }
 
+#define FDMOD_BY_FORM(KERNEL, GRID, BLOCK, ...)                                                     \
+  do {                                                                                             \
+    if (a.form == FDMOD_FORM_COORD) hipLaunchKernelGGL(KERNEL<FDMOD_FORM_COORD>, GRID, BLOCK, 0, s, __VA_ARGS__);   \
+    else if (a.form == FDMOD_FORM_TENSOR)                                                          \
+      hipLaunchKernelGGL(KERNEL<FDMOD_FORM_TENSOR>, GRID, BLOCK, 0, s, __VA_ARGS__);              \
+    else hipLaunchKernelGGL(KERNEL<FDMOD_FORM_BOND>, GRID, BLOCK, 0, s, __VA_ARGS__);              \
+  } while (0)
+
 void launch_adj_ssweep(hipStream_t s, const AdjArgs &a)
 {
-  hipLaunchKernelGGL(k_adj_ssweep, tiles_grid(a.g, a.g.segLV), dim3(64), 0, s, a);
+  FDMOD_BY_FORM(k_adj_ssweep, tiles_grid(a.g, a.g.segLV), dim3(64), a);
 }
 
 void launch_adj_vsweep(hipStream_t s, const AdjArgs &a)
 {
-  hipLaunchKernelGGL(k_adj_vsweep, tiles_grid(a.g, a.g.segLS), dim3(64), 0, s, a);
+  FDMOD_BY_FORM(k_adj_vsweep, tiles_grid(a.g, a.g.segLS), dim3(64), a);
 }
 
 void launch_image(hipStream_t s, const AdjArgs &a, const ImgBlock &b)
 {
   dim3 grid((a.g.izEnd - a.g.izStart + 255) / 256, a.g.ixEnd - a.g.ixStart);
-  if (b.e16[0][0]) hipLaunchKernelGGL(k_image<true>, grid, dim3(256), 0, s, a, b);
-  else hipLaunchKernelGGL(k_image<false>, grid, dim3(256), 0, s, a, b);
+  const bool f16 = b.h16[0][0] != nullptr;
+#define FDMOD_IMG(F)                                                                                \
+  do {                                                                                             \
+    if (f16) hipLaunchKernelGGL((k_image<F, true>), grid, dim3(256), 0, s, a, b);                  \
+    else hipLaunchKernelGGL((k_image<F, false>), grid, dim3(256), 0, s, a, b);                     \
+  } while (0)
+  if (a.form == FDMOD_FORM_COORD) FDMOD_IMG(FDMOD_FORM_COORD);
+  else if (a.form == FDMOD_FORM_TENSOR) FDMOD_IMG(FDMOD_FORM_TENSOR);
+  else FDMOD_IMG(FDMOD_FORM_BOND);
+#undef FDMOD_IMG
 }
 
 // ---------------------------------------------------------------- free surface
@@ -466,7 +681,8 @@ void launch_adj_inject(hipStream_t s, int nr, int ldz, const int *gx, const int 
 // Transpose of the source injection of one step (source_body): the adjoint of
 // src[is][it] is the tap-weighted sum of the adjoint fields the taps wrote into.
 // One block per source; the sum is in double.
-__global__ void k_adj_source(SourceArgs a, const float *ma, const float *mb, double *out)
+__global__ void k_adj_source(SourceArgs a, const float *ma, const float *mb, const float *mc, const float *md,
+                             double *out)
 {
   __shared__ double red[128];
   const Grid &g = a.g;
@@ -480,6 +696,8 @@ __global__ void k_adj_source(SourceArgs a, const float *ma, const float *mb, dou
       const size_t o = (size_t)ix * g.ldz + iz;
       acc += (double)tp.ca * ma[o];
       if (mb) acc += (double)tp.cb * mb[o];
+      if (mc) acc += (double)tp.ca * mc[o];  // tensor form: the stress source also fed tau
+      if (md) acc += (double)tp.cb * md[o];
     }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -490,9 +708,10 @@ __global__ void k_adj_source(SourceArgs a, const float *ma, const float *mb, dou
   if (threadIdx.x == 0) out[(size_t)is * a.nt + a.it] = red[0];
 }
 
-void launch_adj_source(hipStream_t s, const SourceArgs &a, const float *ma, const float *mb, double *out)
+void launch_adj_source(hipStream_t s, const SourceArgs &a, const float *ma, const float *mb, const float *mc,
+                       const float *md, double *out)
 {
-  hipLaunchKernelGGL(k_adj_source, dim3(a.ns), dim3(128), 0, s, a, ma, mb, out);
+  hipLaunchKernelGGL(k_adj_source, dim3(a.ns), dim3(128), 0, s, a, ma, mb, mc, md, out);
 }
 
 // ---------------------------------------------------------------- FP64 reductions
@@ -652,48 +871,102 @@ __device__ void bond_stiffness(D4 vp, double vs, double rho, D4 eps, D4 del, D4 
   }
 }
 
+// Tensor and coordinate forms: the per-point arrays the loop reads (TR:483-517,
+// CR:497-513), VTI stiffness then trig, as exact functions of (vp, eps, delta, theta).
+__device__ int vti_params(int form, D4 vp, double vs, double rho, D4 eps, D4 del, D4 t, D4 c[8])
+{
+  const D4 r = cst(vs) / vp;
+  const D4 f = cst(1.0) - r * r;
+  c[0] = rho * (vp * vp) * (cst(1.0) + 2.0 * eps);
+  c[1] = rho * (vp * vp * dsqrt(f * (f + 2.0 * del)) - cst(vs * vs));
+  c[2] = rho * (vp * vp);
+  c[3] = cst(rho * vs * vs);
+  if (form == FDMOD_FORM_TENSOR) {
+    const D4 sn = dsin(t), co = dcos(t);
+    c[4] = sn * sn;
+    c[5] = co * co;
+    c[6] = dsin(2.0 * t);
+    return 7;
+  }
+  c[4] = dsin(t);
+  c[5] = dcos(t);
+  return 6;
+}
+
 // dJ/d(vp, eps, delta, theta[deg]) on the unpadded model.  Each unpadded cell sums
 // the chain rule over the padded cells that replicate it (the transpose of padModel2d,
-// L/absorbc/padmod.c:33-58).  G holds sum_t mu (x) E; coef2 multiplies it here.
-__global__ void k_chain_bond(Grid g, int nx, int nz, int fs_row, DevArrays d, const double *G, size_t gstride,
-                             double theta_scale, double *grad)
+// L/absorbc/padmod.c:33-58).  Bond: G holds the sums without coef2, multiplied here.
+__global__ void k_chain(Grid g, int form, int nx, int nz, int fs_row, DevArrays d, const double *G, size_t gstride,
+                        double theta_scale, double *grad)
 {
   const int jz = blockIdx.x * blockDim.x + threadIdx.x;
   const int jx = blockIdx.y;
   if (jz >= nz || jx >= nx) return;
   const int x0 = jx == 0 ? 0 : jx + g.e, x1 = jx == nx - 1 ? g.Nx - 1 : jx + g.e;
   const int z0 = jz == 0 ? 0 : jz + g.e, z1 = jz == nz - 1 ? g.Nz - 1 : jz + g.e;
+  const int ng = form == FDMOD_FORM_BOND ? 6 : (form == FDMOD_FORM_TENSOR ? 7 : 6);
   double acc[4] = {0, 0, 0, 0};
   for (int ix = x0; ix <= x1; ix++)
     for (int iz = z0; iz <= z1; iz++) {
       const size_t o = (size_t)ix * g.ldz + iz;
-      double gc[6];
+      double gc[8];
       bool any = false;
-      const double k2 = d.a[FDMOD_A_COEF2][o];
-      for (int j = 0; j < 6; j++) {
+      const double k2 = form == FDMOD_FORM_BOND ? (double)d.a[FDMOD_A_COEF2][o] : 1.0;
+      for (int j = 0; j < ng; j++) {
         gc[j] = k2 * G[j * gstride + o];
         any = any || gc[j] != 0.0;
       }
       if (!any) continue;
-      D4 c[6];
-      bond_stiffness(seed(d.a[FDMOD_A_VP][o], 0), d.a[FDMOD_A_VS][o], d.a[FDMOD_A_RHO][o],
-                     seed(d.a[FDMOD_A_EPS][o], 1), seed(d.a[FDMOD_A_DELTA][o], 2), seed(d.a[FDMOD_A_THETA][o], 3),
-                     iz == fs_row, c);
+      D4 c[8];
+      const D4 vp = seed(d.a[FDMOD_A_VP][o], 0), eps = seed(d.a[FDMOD_A_EPS][o], 1);
+      const D4 del = seed(d.a[FDMOD_A_DELTA][o], 2), th = seed(d.a[FDMOD_A_THETA][o], 3);
+      const double vs = d.a[FDMOD_A_VS][o], rho = d.a[FDMOD_A_RHO][o];
+      if (form == FDMOD_FORM_BOND) bond_stiffness(vp, vs, rho, eps, del, th, iz == fs_row, c);
+      else vti_params(form, vp, vs, rho, eps, del, th, c);
       const double beta = d.a[FDMOD_A_BETA][o];
       for (int m = 0; m < 4; m++) {
         double s = 0.0;
-        for (int j = 0; j < 6; j++) s += gc[j] * c[j].d[m];
+        for (int j = 0; j < ng; j++) s += gc[j] * c[j].d[m];
         acc[m] += m == 0 ? s / beta : (m == 3 ? s * theta_scale : s);
       }
     }
   for (int m = 0; m < 4; m++) grad[((size_t)m * nx + jx) * nz + jz] = acc[m];
 }
 
-void launch_chain_bond(hipStream_t s, const Grid &g, int nx, int nz, int fs_row, const DevArrays &d,
-                       const double *G, size_t gstride, double theta_scale, double *grad)
+void launch_chain(hipStream_t s, const Grid &g, int form, int nx, int nz, int fs_row, const DevArrays &d,
+                  const double *G, size_t gstride, double theta_scale, double *grad)
 {
   dim3 grid((nz + 127) / 128, nx);
-  hipLaunchKernelGGL(k_chain_bond, grid, dim3(128), 0, s, g, nx, nz, fs_row, d, G, gstride, theta_scale, grad);
+  hipLaunchKernelGGL(k_chain, grid, dim3(128), 0, s, g, form, nx, nz, fs_row, d, G, gstride, theta_scale, grad);
+}
+
+// Coordinate form: the velocity-point sums of sinv, cosv (G6, G7) go back to the
+// stress-point sin, cos (G4, G5) through the transpose of the 4-point average of
+// CR:516-535 (sinv(x,z) = (sin(x,z)+sin(x+1,z)+sin(x,z+1)+sin(x+1,z+1))/4 below the
+// last row and column, sin(x,z) on them).  Gather form, one thread per point.
+__global__ void k_fold_trig(Grid g, double *G, size_t gstride)
+{
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ix = blockIdx.y;
+  if (iz >= g.Nz || ix >= g.Nx) return;
+  const size_t o = (size_t)ix * g.ldz + iz;
+  for (int q = 0; q < 2; q++) {
+    const double *gv = G + (6 + q) * gstride;
+    double acc = 0.0;
+    for (int dx = 0; dx <= 1; dx++)
+      for (int dz = 0; dz <= 1; dz++) {
+        const int px = ix - dx, pz = iz - dz;
+        if (px >= 0 && pz >= 0 && px < g.Nx - 1 && pz < g.Nz - 1) acc += 0.25 * gv[(size_t)px * g.ldz + pz];
+      }
+    if (ix == g.Nx - 1 || iz == g.Nz - 1) acc += gv[o];
+    G[(4 + q) * gstride + o] += acc;
+  }
+}
+
+void launch_fold_trig(hipStream_t s, const Grid &g, double *G, size_t gstride)
+{
+  dim3 grid((g.Nz + 255) / 256, g.Nx);
+  hipLaunchKernelGGL(k_fold_trig, grid, dim3(256), 0, s, g, G, gstride);
 }
 
 // Add one shot's crop gradient [4][nx][nz] into the model gradient [4][nx_full][nz]

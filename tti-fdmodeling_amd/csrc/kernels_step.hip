@@ -45,7 +45,7 @@ __device__ __forceinline__ void load_vops(VOps &p, const SweepArgs &a, size_t o,
 // Coordinate (COORD): CR:614-671, with the reference's invdx on d(sxx)/dz.
 // The loads of the staged column and of the per-point operands run PD iterations
 // ahead of their use.
-template <bool COORD, int PD>
+template <bool COORD, int PD, int ESTORE = 0>
 __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx)
 {
   __shared__ float ring[3][kRing][LZ];
@@ -88,6 +88,7 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
       if (zok && k + PD < nc) load_vops<COORD>(on, a, (size_t)(c + PD) * g.ldz + iz, und, g.dt);
       store_col<3>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
+      float hv[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // coordinate-form history (ESTORE)
       if (zok) {
         const VOps &op = opq[0];
         float d1, d2;
@@ -109,8 +110,15 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
           const float cv = op.cv, sv = op.sv;
           st_out(a.vx + o, op.k1 * op.vx + kr * (cv * sxxdx - sv * sxxdz + sv * sxzdx + cv * sxzdz));
           st_out(a.vz + o, op.k1 * op.vz + kr * (cv * sxzdx - sv * sxzdz + sv * szzdx + cv * szzdz));
+          if (ESTORE) {
+            hv[0] = -sxxdz + sxzdx;
+            hv[1] = -sxzdz + szzdx;
+            hv[2] = sxxdx + sxzdz;
+            hv[3] = sxzdx + szzdz;
+          }
         }
       }
+      if (ESTORE) store_hist<ESTORE, 4>(a, (size_t)c * g.ldz + iz, c, btz, zok, hv, 4, 1);
 #pragma unroll
       for (int i = 0; i + 1 < PD; i++) {
         colq[i] = colq[i + 1];
@@ -152,20 +160,21 @@ __device__ __forceinline__ void load_sops(SOps &p, const SweepArgs &a, size_t o,
 //   FORM_BOND   BT:836-873
 //   FORM_TENSOR TR:799-836 and the sigma <- R(tau) sweep TR:839-852, fused pointwise
 //   FORM_COORD  CR:804-845
-// Store one column of the strain fields for the gradient (all lanes call it).
-template <int ESTORE>
-__device__ __forceinline__ void store_strain(const SweepArgs &a, size_t o, int c, int btz, bool zok, float e0,
-                                             float e1, float e2)
+// Store one column of a group of history fields for the gradient (all lanes call it;
+// rows past izEnd pass zeros).  FP16: one scale for the group's fields.
+template <int ESTORE, int NF>
+__device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, int btz, bool zok, const float (&e)[NF],
+                                           int f0, int grp)
 {
-  const float e[3] = {e0, e1, e2};
   if (ESTORE == 1) {
     if (zok)
 #pragma unroll
-      for (int f = 0; f < 3; f++) a.e32[f][o] = e[f];
+      for (int f = 0; f < NF; f++) a.h32[f0 + f][o] = e[f];
     return;
   }
-  // one scale for the three strain components of the column block (same units)
-  float m = fmaxf(fabsf(e0), fmaxf(fabsf(e1), fabsf(e2)));
+  float m = 0.0f;
+#pragma unroll
+  for (int f = 0; f < NF; f++) m = fmaxf(m, fabsf(e[f]));
   for (int s = 32; s > 0; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
   int E = 0;
   if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
@@ -173,8 +182,8 @@ __device__ __forceinline__ void store_strain(const SweepArgs &a, size_t o, int c
   const float up = ldexpf(1.0f, 14 - E);
   if (zok)
 #pragma unroll
-    for (int f = 0; f < 3; f++) a.e16[f][o] = (_Float16)(e[f] * up);
-  if (threadIdx.x == 0) a.esc[0][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
+    for (int f = 0; f < NF; f++) a.h16[f0 + f][o] = (_Float16)(e[f] * up);
+  if (threadIdx.x == 0) a.hsc[grp][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
 }
 
 template <int FORM, int PD, int ESTORE>
@@ -222,7 +231,7 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
       __syncthreads();
       const size_t o = (size_t)c * g.ldz + iz;
       float nxx = 0.0f, nzz = 0.0f, nxz = 0.0f;
-      float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+      float hv[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // history values (ESTORE)
       if (zok) {
         const SOps &op = opq[0];
         float d1, d2;
@@ -232,10 +241,15 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
         rsg_bwd([&](int dx, int dz) { return ring[1][(u + dx + 5 + kRing) % kRing][lane + 5 + dz]; }, d1, d2);
         float vzdx = 0.5f * (d1 + d2) * invdx;
         float vzdz = 0.5f * (d1 - d2) * invdz;
-        if (ESTORE) {
-          e0 = vxdx;
-          e1 = vzdz;
-          e2 = vzdx + vxdz;
+        if (ESTORE && FORM != FDMOD_FORM_COORD) {
+          hv[0] = vxdx;
+          hv[1] = vzdz;
+          hv[2] = vzdx + vxdz;
+        } else if (ESTORE) {
+          hv[0] = vxdx;
+          hv[1] = vxdz;
+          hv[2] = vzdx;
+          hv[3] = vzdz;
         }
         const float k1 = op.k1, k2 = op.k2;
         if (FORM == FDMOD_FORM_BOND) {
@@ -265,6 +279,9 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
           st_out(a.txx + o, t1);
           st_out(a.tzz + o, t2);
           st_out(a.txz + o, t3);
+          hv[3] = t1;
+          hv[4] = t2;
+          hv[5] = t3;
           nxx = K2 * t1 + S2 * t2 + S2T * t3;
           nzz = S2 * t1 + K2 * t2 - S2T * t3;
           nxz = -0.5f * S2T * (t1 - t2) + (K2 - S2) * t3;
@@ -293,7 +310,19 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
         st_out(a.szz + o, nzz);
         st_out(a.sxz + o, nxz);
       }
-      if (ESTORE) store_strain<ESTORE>(a, o, c, btz, zok, e0, e1, e2);
+      if (ESTORE) {
+        if (FORM == FDMOD_FORM_BOND) {
+          const float e[3] = {hv[0], hv[1], hv[2]};
+          store_hist<ESTORE, 3>(a, o, c, btz, zok, e, 0, 0);
+        } else if (FORM == FDMOD_FORM_TENSOR) {
+          const float e[3] = {hv[0], hv[1], hv[2]}, t[3] = {hv[3], hv[4], hv[5]};
+          store_hist<ESTORE, 3>(a, o, c, btz, zok, e, 0, 0);
+          store_hist<ESTORE, 3>(a, o, c, btz, zok, t, 3, 1);
+        } else {
+          const float e[4] = {hv[0], hv[1], hv[2], hv[3]};
+          store_hist<ESTORE, 4>(a, o, c, btz, zok, e, 0, 0);
+        }
+      }
 #pragma unroll
       for (int i = 0; i + 1 < PD; i++) {
         colq[i] = colq[i + 1];
@@ -315,12 +344,12 @@ constexpr int kPipeV = FDMOD_PIPE_V;  // prefetch depth (iterations) of the velo
 constexpr int kPipeS = FDMOD_PIPE_S;  // and of the stress sweep
 
 // Single grid: blocks map to tiles of one shot.
-template <bool COORD, int PD>
+template <bool COORD, int PD, int ESTORE = 0>
 __global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
 {
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLV, btz, btx)) return;
-  vsweep_tile<COORD, PD>(a, btz, btx);
+  vsweep_tile<COORD, PD, ESTORE>(a, btz, btx);
 }
 
 template <int FORM, int PD, int ESTORE = 0>
@@ -385,18 +414,28 @@ int sweep_tiles(const Grid &g, int segL) { return g.tilesZ * ((g.ixEnd - g.ixSta
 void launch_vsweep(hipStream_t s, int form, const SweepArgs &a)
 {
   dim3 grid = sweep_grid(a.g, a.g.segLV);
-  if (form == FDMOD_FORM_COORD) hipLaunchKernelGGL((k_vsweep<true, kPipeV>), grid, dim3(64), 0, s, a);
-  else hipLaunchKernelGGL((k_vsweep<false, kPipeV>), grid, dim3(64), 0, s, a);
+  if (form == FDMOD_FORM_COORD) {
+    if (a.estore == 1) hipLaunchKernelGGL((k_vsweep<true, kPipeV, 1>), grid, dim3(64), 0, s, a);
+    else if (a.estore == 2) hipLaunchKernelGGL((k_vsweep<true, kPipeV, 2>), grid, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_vsweep<true, kPipeV>), grid, dim3(64), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_vsweep<false, kPipeV>), grid, dim3(64), 0, s, a);
+  }
 }
 
 void launch_ssweep(hipStream_t s, int form, const SweepArgs &a)
 {
   dim3 grid = sweep_grid(a.g, a.g.segLS);
-  if (form == FDMOD_FORM_COORD) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_COORD, kPipeS>), grid, dim3(64), 0, s, a);
-  else if (form == FDMOD_FORM_TENSOR) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_TENSOR, kPipeS>), grid, dim3(64), 0, s, a);
-  else if (a.estore == 1) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_BOND, kPipeS, 1>), grid, dim3(64), 0, s, a);
-  else if (a.estore == 2) hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_BOND, kPipeS, 2>), grid, dim3(64), 0, s, a);
-  else hipLaunchKernelGGL((k_ssweep<FDMOD_FORM_BOND, kPipeS>), grid, dim3(64), 0, s, a);
+#define FDMOD_SSWEEP(F)                                                                            \
+  do {                                                                                           \
+    if (a.estore == 1) hipLaunchKernelGGL((k_ssweep<F, kPipeS, 1>), grid, dim3(64), 0, s, a);    \
+    else if (a.estore == 2) hipLaunchKernelGGL((k_ssweep<F, kPipeS, 2>), grid, dim3(64), 0, s, a); \
+    else hipLaunchKernelGGL((k_ssweep<F, kPipeS>), grid, dim3(64), 0, s, a);                     \
+  } while (0)
+  if (form == FDMOD_FORM_COORD) FDMOD_SSWEEP(FDMOD_FORM_COORD);
+  else if (form == FDMOD_FORM_TENSOR) FDMOD_SSWEEP(FDMOD_FORM_TENSOR);
+  else FDMOD_SSWEEP(FDMOD_FORM_BOND);
+#undef FDMOD_SSWEEP
 }
 
 void launch_vsweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total)

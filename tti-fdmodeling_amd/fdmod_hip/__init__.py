@@ -151,7 +151,7 @@ class Modeling:
 
     def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, ncheckpoints=0, storage="f32", history=False):
         """J = 1/2 |d - obs|^2 over the given components and dJ/d(vp, eps, delta, theta[deg])
-        as a float64 array (4, nx, nz); runs the forward from rest (Bond form).
+        as a float64 array (4, nx, nz); runs the forward from rest.
         storage "f32" or "f16" (stored wavefields); history keeps every step's velocity
         instead of checkpoints."""
         keep, ptrs = self._series((obs_vx, obs_vz, obs_p))
@@ -263,7 +263,7 @@ class Batch:
 
     def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, ncheckpoints=0, storage="f32", history=False,
                  x0=None, nx_full=0):
-        """Misfit gradients of all shots (Bond form).  obs_* are lists of per-shot
+        """Misfit gradients of all shots.  obs_* are lists of per-shot
         (ntrace, ntsample) arrays (None = component absent).  With x0 (per-shot first
         model column) the result is the FP64 sum over shots on the nx_full-column model,
         shape (4, nx_full, nz); otherwise a list of per-shot (4, nx[s], nz) arrays.

@@ -1,4 +1,4 @@
-"""torch.autograd.Function over the libfdmod_hip misfit gradient (Bond form).
+"""torch.autograd.Function over the libfdmod_hip misfit gradient (all three forms).
 
 J(vp, eps, delta, theta) = 1/2 sum (d(m) - obs)^2 for one shot, where d(m) is the
 modelled data of a fdmod_hip.Modeling context (source and receivers set) and the
