@@ -73,6 +73,7 @@ static void release(fdmod_ctx *c)
   dfree(c->pool); dfree(c->flags);
   dfree(c->d_isx); dfree(c->d_isz); dfree(c->d_src); dfree(c->d_taps);
   dfree(c->d_gx); dfree(c->d_gz); dfree(c->d_rvx); dfree(c->d_rvz); dfree(c->d_rp);
+  dfree(c->d_rc_start); dfree(c->d_rc_z); dfree(c->d_rc_id);
   dfree(c->d_p); dfree(c->d_E); dfree(c->d_Ek); dfree(c->d_Ep); dfree(c->d_partial);
   dfree(c->d_mu); dfree(c->d_G); dfree(c->d_snap); dfree(c->d_res); dfree(c->d_red);
   dfree(c->ckpt.data); dfree(c->ckpt.scale); dfree(c->hist.data); dfree(c->hist.scale);
@@ -332,6 +333,18 @@ extern "C" int fdmod_set_source(fdmod_ctx *c, int stype, int ns, const int *posx
   c->d_isx = c->d_isz = nullptr; c->d_src = nullptr; c->d_taps = nullptr;
   c->stype = stype;
   c->ns = ns;
+  // the stress sweep can add a single explosive source whose footprint it writes:
+  // rows down from the image rows (free surface) or from izStart, columns of U
+  {
+    const Grid &g = c->g;
+    const int top = g.izStart - (p.freesurface ? kN : 0);
+    c->fuse_src = ns == 1 && stype == 1 && p.form != FDMOD_FORM_TENSOR && isx[0] - kN >= g.ixStart &&
+                  isx[0] + kN < g.ixEnd && isz[0] - kN >= top && isz[0] + kN < g.izEnd;
+    if (c->fuse_src) {
+      c->src_cell_x = isx[0];
+      c->src_cell_z = isz[0];
+    }
+  }
   if (ns == 0) return FDMOD_OK;
   HIPCALL(c, hipMalloc(&c->d_isx, ns * sizeof(int)));
   HIPCALL(c, hipMalloc(&c->d_isz, ns * sizeof(int)));
@@ -360,7 +373,9 @@ extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, c
                   posx[i], p.nx);
   }
   dfree(c->d_gx); dfree(c->d_gz); dfree(c->d_rvx); dfree(c->d_rvz); dfree(c->d_rp);
+  dfree(c->d_rc_start); dfree(c->d_rc_z); dfree(c->d_rc_id);
   c->d_gx = c->d_gz = nullptr; c->d_rvx = c->d_rvz = c->d_rp = nullptr;
+  c->d_rc_start = c->d_rc_z = c->d_rc_id = nullptr;
   c->seistype = seistype;
   c->mt = mt;
   c->nr = nr;
@@ -385,6 +400,24 @@ extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, c
   if (seistype == 2 || seistype == 3) {
     HIPCALL(c, hipMalloc(&c->d_rp, nrec));
     HIPCALL(c, hipMemsetAsync(c->d_rp, 0, nrec, c->stream));
+  }
+  if (seistype == 2) {  // pressure receivers by padded column, for the stress sweep to record
+    const int Nx = c->g.Nx;
+    std::vector<int> start(Nx + 1, 0), rz(nr), rid(nr);
+    for (int i = 0; i < nr; i++) start[gx[i] + 1]++;
+    for (int x = 0; x < Nx; x++) start[x + 1] += start[x];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < nr; i++) {
+      const int k = fill[gx[i]]++;
+      rz[k] = gz[i];
+      rid[k] = i;
+    }
+    HIPCALL(c, hipMalloc(&c->d_rc_start, (Nx + 1) * sizeof(int)));
+    HIPCALL(c, hipMalloc(&c->d_rc_z, nr * sizeof(int)));
+    HIPCALL(c, hipMalloc(&c->d_rc_id, nr * sizeof(int)));
+    HIPCALL(c, hipMemcpy(c->d_rc_start, start.data(), (Nx + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCALL(c, hipMemcpy(c->d_rc_z, rz.data(), nr * sizeof(int), hipMemcpyHostToDevice));
+    HIPCALL(c, hipMemcpy(c->d_rc_id, rid.data(), nr * sizeof(int), hipMemcpyHostToDevice));
   }
   HIPCALL(c, hipStreamSynchronize(c->stream));
   return FDMOD_OK;
@@ -492,11 +525,34 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
     src.fb = (c->stype == 5 || c->stype == 6) ? nullptr : sa.vz;
     launch_source(c->stream, src);
   }
-  launch_ssweep(c->stream, p.form, sa);
   const bool snap_now = record && c->wsave > 0 && it % c->snap_sample == 0;
   const bool rec_now = record && c->seistype > 0 && c->nr > 0 && it % c->mt == 0;
+  const bool fsrc = c->fuse_src && c->ns > 0;
+  const bool frec = rec_now && c->d_rc_start && p.form != FDMOD_FORM_TENSOR;
+  if (fsrc || frec) {  // the stress sweep adds the source and/or records the receivers
+    SweepArgs sf = sa;
+    if (fsrc) {
+      sf.fuse |= 1;
+      sf.fsx = c->src_cell_x;
+      sf.fsz = c->src_cell_z;
+      sf.ftap = c->d_taps;
+      sf.fsrc = c->d_src;
+      sf.fit = it;
+    }
+    if (frec) {
+      sf.fuse |= 2;
+      sf.rc_start = c->d_rc_start;
+      sf.rc_z = c->d_rc_z;
+      sf.rc_id = c->d_rc_id;
+      sf.rec_p = c->d_rp;
+      sf.rec_off = (size_t)(it / c->mt) * c->nr;
+    }
+    launch_ssweep(c->stream, p.form, sf);
+  } else {
+    launch_ssweep(c->stream, p.form, sa);
+  }
   RecvArgs r{};
-  if (rec_now) {
+  if (rec_now && !frec) {
     r.nr = c->nr; r.ldz = c->g.ldz; r.gx = c->d_gx; r.gz = c->d_gz;
     r.vx = sa.vx; r.vz = sa.vz; r.sxx = sa.sxx; r.szz = sa.szz;
     r.out_vx = c->d_rvx; r.out_vz = c->d_rvz; r.out_p = c->d_rp;
@@ -505,7 +561,7 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
     r.want_p = c->seistype == 2 || c->seistype == 3;
     r.mt = c->mt;
   }
-  if (c->ns > 0 && c->stype == 1) {
+  if (c->ns > 0 && c->stype == 1 && !fsrc) {
     src.fa = sa.sxx;
     src.fb = sa.szz;
     if (p.form == FDMOD_FORM_TENSOR) {  // TR:888-891
@@ -519,7 +575,7 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
     int rc = snapshot(c, it);
     if (rc) return rc;
   }
-  if (rec_now) launch_receivers(c->stream, r);
+  if (rec_now && !frec) launch_receivers(c->stream, r);
   if (c->energy) {
     launch_energy_partial(c->stream, c->g, p.nx, p.nz, c->d, c->d_partial, c->eblocks);
     launch_energy_final(c->stream, c->d_partial, c->eblocks, p.dx, p.dz, c->d_E, c->d_Ek, c->d_Ep, it);

@@ -32,8 +32,11 @@ struct fdmod_ctx {
   int *d_isx = nullptr, *d_isz = nullptr;
   float *d_src = nullptr;
   fdm::SrcTap *d_taps = nullptr;
+  int fuse_src = 0;  // the stress sweep adds the stress source (one stype-1 source it covers)
+  int src_cell_x = 0, src_cell_z = 0;  // its padded cell
   // receivers
   int seistype = 0, mt = 1, nr = 0, ntsample = 0;
+  int *d_rc_start = nullptr, *d_rc_z = nullptr, *d_rc_id = nullptr;  // receivers by column (CSR)
   int *d_gx = nullptr, *d_gz = nullptr;
   float *d_rvx = nullptr, *d_rvz = nullptr, *d_rp = nullptr;
   // snapshots

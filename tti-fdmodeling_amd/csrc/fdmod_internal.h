@@ -70,6 +70,17 @@ struct SweepArgs {
   float *h32[8];
   _Float16 *h16[8];
   float *hsc[2];
+  // Single-shot loop, Bond and coordinate forms: the stress sweep also adds the stress
+  // source (fuse bit 1: stype 1, one source whose 11 x 11 footprint the sweep writes;
+  // ftap = its tap table, ordered ox-major) and records the pressure receivers (bit 2:
+  // rc_* = receivers by padded column, CSR; rec_p[rec_off + id]).
+  int fuse;
+  int fsx, fsz, fit;
+  const struct SrcTap *ftap;
+  const float *fsrc;
+  const int *rc_start, *rc_z, *rc_id;
+  float *rec_p;
+  size_t rec_off;
 };
 
 // Source injection table entry: offset from the source cell and the two

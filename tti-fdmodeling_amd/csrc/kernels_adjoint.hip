@@ -370,7 +370,7 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
         const size_t o = (size_t)c * g.ldz + iz;
         const SAdjOps &op = opq[0];
         float gxx, gzz, gxz;  // adjoint contributions to subtract
-        if (FORM != FDMOD_FORM_COORD) {
+        if constexpr (FORM != FDMOD_FORM_COORD) {
           const float nxdx = 0.5f * (dd1[0] + dd2[0]) * invdx, nxdz = 0.5f * (dd1[0] - dd2[0]) * invdz;
           const float nzdx = 0.5f * (dd1[1] + dd2[1]) * invdx, nzdz = 0.5f * (dd1[1] - dd2[1]) * invdz;
           gxx = nxdx;

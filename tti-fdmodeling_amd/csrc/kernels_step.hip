@@ -20,6 +20,32 @@
 
 namespace fdm {
 
+// Store one column of a group of history fields for the gradient (all lanes call it;
+// rows past izEnd pass zeros).  FP16: one scale for the group's fields.
+template <int ESTORE, int NF>
+__device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, int btz, bool zok, const float (&e)[NF],
+                                           int f0, int grp)
+{
+  if (ESTORE == 1) {
+    if (zok)
+#pragma unroll
+      for (int f = 0; f < NF; f++) a.h32[f0 + f][o] = e[f];
+    return;
+  }
+  float m = 0.0f;
+#pragma unroll
+  for (int f = 0; f < NF; f++) m = fmaxf(m, fabsf(e[f]));
+  for (int s = 32; s > 0; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+  int E = 0;
+  if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
+  E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
+  const float up = ldexpf(1.0f, 14 - E);
+  if (zok)
+#pragma unroll
+    for (int f = 0; f < NF; f++) a.h16[f0 + f][o] = (_Float16)(e[f] * up);
+  if (threadIdx.x == 0) a.hsc[grp][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
+}
+
 // ---------------------------------------------------------------- velocity sweep
 struct VOps {
   float vx, vz, ir, k1, k2, cv, sv;
@@ -160,32 +186,6 @@ __device__ __forceinline__ void load_sops(SOps &p, const SweepArgs &a, size_t o,
 //   FORM_BOND   BT:836-873
 //   FORM_TENSOR TR:799-836 and the sigma <- R(tau) sweep TR:839-852, fused pointwise
 //   FORM_COORD  CR:804-845
-// Store one column of a group of history fields for the gradient (all lanes call it;
-// rows past izEnd pass zeros).  FP16: one scale for the group's fields.
-template <int ESTORE, int NF>
-__device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, int btz, bool zok, const float (&e)[NF],
-                                           int f0, int grp)
-{
-  if (ESTORE == 1) {
-    if (zok)
-#pragma unroll
-      for (int f = 0; f < NF; f++) a.h32[f0 + f][o] = e[f];
-    return;
-  }
-  float m = 0.0f;
-#pragma unroll
-  for (int f = 0; f < NF; f++) m = fmaxf(m, fabsf(e[f]));
-  for (int s = 32; s > 0; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
-  int E = 0;
-  if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
-  E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
-  const float up = ldexpf(1.0f, 14 - E);
-  if (zok)
-#pragma unroll
-    for (int f = 0; f < NF; f++) a.h16[f0 + f][o] = (_Float16)(e[f] * up);
-  if (threadIdx.x == 0) a.hsc[grp][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
-}
-
 template <int FORM, int PD, int ESTORE>
 __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx)
 {
@@ -292,24 +292,53 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
         nzz = a.szz[o];
         nxz = a.sxz[o];
       }
+      // free-surface images of row izStart+lane into row izStart-lane (lanes 1..kN)
+      const bool img = fs_top && lane >= 1 && lane <= kN;
+      float ixx = 0.0f, izz = 0.0f, ixz = 0.0f;
       if (fs_top) {
         if (lane == 0) {
           nxz = 0.0f;
           nzz = 0.0f;
         }
         const float s0 = __shfl(nxx, 0, 64);
-        if (lane >= 1 && lane <= kN) {
-          const size_t m = (size_t)c * g.ldz + (z0 - lane);
-          a.sxx[m] = 2.0f * s0 - nxx;
-          a.szz[m] = -nzz;
-          a.sxz[m] = -nxz;
+        if (img) {
+          ixx = 2.0f * s0 - nxx;
+          izz = -nzz;
+          ixz = -nxz;
         }
+      }
+      if (a.fuse & 1) {  // stress source (BT:893-914) after the images, tap by tap as k_source adds it
+        const int ox = c - a.fsx;
+        if (ox >= -kN && ox <= kN) {
+          const float sv = a.fsrc[a.fit];
+          const SrcTap *tp = a.ftap + (ox + kN) * (2 * kN + 1);
+          const int oz = iz - a.fsz, om = z0 - lane - a.fsz;
+          if (zok && oz >= -kN && oz <= kN) {
+            const float da = tp[oz + kN].ca * sv;
+            nxx = nxx + da;
+            nzz = nzz + da;
+          }
+          if (img && om >= -kN && om <= kN) {
+            const float da = tp[om + kN].ca * sv;
+            ixx = ixx + da;
+            izz = izz + da;
+          }
+        }
+      }
+      if (img) {
+        const size_t m = (size_t)c * g.ldz + (z0 - lane);
+        a.sxx[m] = ixx;
+        a.szz[m] = izz;
+        a.sxz[m] = ixz;
       }
       if (zok) {
         st_out(a.sxx + o, nxx);
         st_out(a.szz + o, nzz);
         st_out(a.sxz + o, nxz);
       }
+      if (a.fuse & 2)  // pressure receivers of this column (isoelastic.c:103-144)
+        for (int j = a.rc_start[c]; j < a.rc_start[c + 1]; j++)
+          if (zok && a.rc_z[j] == iz) a.rec_p[a.rec_off + a.rc_id[j]] = 0.5f * (nxx + nzz);
       if (ESTORE) {
         if (FORM == FDMOD_FORM_BOND) {
           const float e[3] = {hv[0], hv[1], hv[2]};
