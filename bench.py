@@ -200,7 +200,8 @@ def _rel(a, b):
 
 def gradient_single(full, true_full, nt):
     """Adjoint gradient workload: one shot on a 1126 x 2000 crop, nt steps; the plain
-    FP32 checkpointed gradient and the FP16 full-history gradient."""
+    FP32 checkpointed gradient, the FP16 full-history gradient and the FP32
+    boundary-saving reconstruction."""
     from fdmod_hip import Modeling, synth
 
     W, x0 = 2000, 2936
@@ -214,7 +215,8 @@ def gradient_single(full, true_full, nt):
                        f"true model = background + 5 Gaussian anomalies, nt={nt}, dt=0.8 ms, no free surface, "
                        f"{W} pressure receivers; gradient wrt vp, eps, delta, theta"}
     res = {}
-    for name, kw in (("plain_f32_checkpoints", dict()), ("f16_history", dict(storage="f16", history=True))):
+    for name, kw in (("plain_f32_checkpoints", dict()), ("f16_history", dict(storage="f16", history=True)),
+                     ("f32_reconstruct", dict(history="reconstruct"))):
         g.gradient(obs_p=obs, **kw)  # warm-up (allocations)
         gpu_sync()
         t0 = time.perf_counter()
@@ -223,6 +225,8 @@ def gradient_single(full, true_full, nt):
         res[name] = gr
         out[name] = {"seconds": round(wall, 4), "gradients_per_hour": round(3600.0 / wall, 1), "misfit": J}
     out["f16_vs_f32_rel_l2"] = [round(x, 7) for x in _rel(res["f16_history"], res["plain_f32_checkpoints"])]
+    out["reconstruct_vs_f32_rel_l2"] = [float("%.3g" % x)
+                                        for x in _rel(res["f32_reconstruct"], res["plain_f32_checkpoints"])]
     g.close()
     return out
 

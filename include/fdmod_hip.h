@@ -187,9 +187,20 @@ int fdmod_gradient(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, con
  * E(v_{k+1}) = (vxdx, vzdz, vzdx + vxdz) the stress sweep computes at every step,
  * which is all the imaging needs.  FP16 stores each field as FP16 with one
  * power-of-two scale per column and 64-row block (stencil arithmetic stays FP32).
- * history = 1 keeps the strain of every step (3*nt fields, no recomputation);
- * history = 0 keeps ncheckpoints states and recomputes each segment's strain.
- * fdmod_gradient is {ncheckpoints, FDMOD_STORE_F32, 0}. */
+ * history selects how the backward pass gets the forward wavefield:
+ *   FDMOD_HIST_CHECKPOINTS   ncheckpoints states, each segment recomputed with its strain;
+ *   FDMOD_HIST_FULL          the strain of every step (3*nt fields, no recomputation);
+ *   FDMOD_HIST_RECONSTRUCT   boundary-saving reconstruction: the forward saves the state on
+ *                            a band 5 points wide around the undamped interior and the
+ *                            strain outside it, every step; the backward pass runs the loop
+ *                            backwards in time over the interior (dt negated), putting the
+ *                            band back after each sweep.  ncheckpoints > 0 also stores that
+ *                            many full states and restarts the reversed run from each (0:
+ *                            none).  FP32 only (storage must be FDMOD_STORE_F32).
+ * fdmod_gradient is {ncheckpoints, FDMOD_STORE_F32, FDMOD_HIST_CHECKPOINTS}. */
+#define FDMOD_HIST_CHECKPOINTS 0
+#define FDMOD_HIST_FULL 1
+#define FDMOD_HIST_RECONSTRUCT 2
 #define FDMOD_STORE_F32 0
 #define FDMOD_STORE_F16 1      /* strain FP16, checkpoints FP32 */
 #define FDMOD_STORE_F16_ALL 2  /* strain and checkpoints FP16: restarts from rounded
@@ -198,7 +209,7 @@ int fdmod_gradient(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, con
 typedef struct fdmod_grad_opts {
   int ncheckpoints; /* <= 0: ceil(sqrt(nt))                    */
   int storage;      /* FDMOD_STORE_F32, _F16 or _F16_ALL        */
-  int history;      /* 0: checkpoints + recomputation; 1: strain of every step */
+  int history;      /* FDMOD_HIST_CHECKPOINTS, _FULL or _RECONSTRUCT */
 } fdmod_grad_opts;
 int fdmod_gradient_opts(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, const float *obs_p,
                         const fdmod_grad_opts *opts, double *misfit, double *grad);

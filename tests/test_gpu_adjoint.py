@@ -180,3 +180,46 @@ def test_dot_product_at_gradient_workload_size():
     y = np.random.default_rng(5).standard_normal((2000, nt)).astype(np.float32)
     lhs, rhs = g.adjoint_dot(None, None, y)
     assert np.isfinite(lhs) and abs(lhs - rhs) <= 1e-4 * abs(lhs), (lhs, rhs)
+
+
+def _obs_src(fs, stype, seis, mode, form):
+    m = _model()
+    true = {k: v.copy() for k, v in m.items()}
+    true["vp"] = (true["vp"] * 1.03).astype(np.float32)
+    true["theta"] = (true["theta"] + 4).astype(np.float32)
+    ot = _mk(Oracle, true, fs, stype=stype, seis=seis, mode=mode, form=form, prec="f64")
+    ot.run()
+    use = _use(seis)
+    return m, [ot.traces(c).astype(np.float32) if use[c] else None for c in (1, 3, 4)]
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("fs,stype,seis,mode", [(0, 1, 2, 1), (1, 1, 3, 1), (1, 2, 3, 2), (0, 4, 1, 1)])
+def test_gradient_boundary_saving_reconstruction(fs, stype, seis, mode, form):
+    # The loop run backwards over the undamped interior (saved band states, saved
+    # history outside the interior) rebuilds the forward wavefield to FP32 rounding:
+    # the gradient matches the checkpointed one and the FP64 oracle's.
+    print("reconstruct", form, fs, stype, seis, mode, flush=True)
+    m, obs = _obs_src(fs, stype, seis, mode, form)
+    g = _mk(Modeling, m, fs, stype=stype, seis=seis, mode=mode, form=form)
+    Jc, gc = g.gradient(*obs)
+    J, gr = g.gradient(*obs, history="reconstruct")
+    assert J == Jc  # same forward arithmetic
+    errs = [rel_l2(gr[i], gc[i]) for i in range(4)]
+    print("  vs checkpoints", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) <= 1e-5, errs
+    J, ga = g.gradient(*obs, ncheckpoints=7, history="reconstruct")  # with restart states
+    assert J == Jc and max(rel_l2(ga[i], gc[i]) for i in range(4)) <= 1e-5
+    o = _mk(Oracle, m, fs, stype=stype, seis=seis, mode=mode, form=form, prec="f64")
+    Jr, gref = o.gradient(*obs)
+    for i in range(4):
+        assert rel_l2(gr[i], gref[i]) <= 1e-4, i
+
+
+def test_reconstruction_rejects_fp16_storage():
+    from fdmod_hip import FdmodError
+
+    m, obs = _obs(0, 2)
+    g = _mk(Modeling, m, 0, seis=2)
+    with pytest.raises(FdmodError, match="FP32"):
+        g.gradient(*obs, storage="f16", history="reconstruct")

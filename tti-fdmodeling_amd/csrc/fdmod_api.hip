@@ -77,7 +77,7 @@ static void release(fdmod_ctx *c)
   dfree(c->d_p); dfree(c->d_E); dfree(c->d_Ek); dfree(c->d_Ep); dfree(c->d_partial);
   dfree(c->d_mu); dfree(c->d_G); dfree(c->d_snap); dfree(c->d_res); dfree(c->d_red);
   dfree(c->ckpt.data); dfree(c->ckpt.scale); dfree(c->hist.data); dfree(c->hist.scale);
-  dfree(c->d_srcadj); dfree(c->d_grad);
+  dfree(c->d_srcadj); dfree(c->d_grad); dfree(c->d_coef2n); dfree(c->d_srcn);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -59,6 +59,9 @@ struct fdmod_ctx {
   double *d_red = nullptr;     // reduction partials (kRedBlocks) and scalars
   double *d_srcadj = nullptr;  // adjoint of the source time functions [ns][nt]
   double *d_grad = nullptr;    // gradient [4][nx][nz]
+  float *d_coef2n = nullptr;   // -coef2 (Nx*ldz) and -src ([ns][nt]) for time-reversed steps
+  float *d_srcn = nullptr;
+  size_t d_srcn_n = 0;
   std::string err;
 };
 

@@ -10,6 +10,8 @@
 //   3. for each checkpoint segment, last first: restore its state and recompute its
 //      steps storing the strain of each (skipped with the full history), then run
 //      the adjoint steps of the segment backwards, imaging with the stored strain;
+//      with boundary-saving reconstruction, each adjoint step is preceded by the
+//      time-reversed loop iteration, which rebuilds the state and stores the strain;
 //   4. chain rule from the six stiffness sums to (vp, eps, delta, theta) and the
 //      transpose of the edge padding.
 // Stored fields are FP32 copies or FP16 with per-(column, 64-row block) scales.
@@ -267,6 +269,115 @@ int state_io(fdmod_ctx *c, size_t slot, bool f16, bool save)
 }
 
 
+// ---- boundary-saving reconstruction ----
+// R is the undamped interior (coef1 = 1, coef2 = dt; under a free surface, the rows
+// below the surface row), where a loop iteration can be undone: with dt and coef2
+// negated the two sweeps subtract what they added, and the sources come off with
+// negated series.  A point of R reads the other family of fields up to kN points
+// away, so the state on the band of width kN around R is saved at every step and put
+// back after each reversed sweep.  Outside R the reversed fields are not the
+// forward's, so the history of the frame U \ R (U = the update range) is saved by
+// the forward and written over the history the reversed sweeps store there.
+struct Recon {
+  int rx0 = 0, rx1 = 0, rz0 = 0, rz1 = 0;
+  FrameIO band{}, frame{};  // rectangles only; fields, pack and direction set per use
+  size_t nband = 0, nframe = 0;
+};
+
+void set_rects(FrameIO &f, const int (*r)[4], int n, int ldz)
+{
+  f = FrameIO{};
+  f.ldz = ldz;
+  long long off = 0;
+  for (int i = 0; i < n; i++) {
+    const int x0 = r[i][0], x1 = r[i][1], z0 = r[i][2], z1 = r[i][3];
+    if (x1 <= x0 || z1 <= z0) continue;
+    f.x0[f.nrect] = x0;
+    f.z0[f.nrect] = z0;
+    f.h[f.nrect] = z1 - z0;
+    f.off[f.nrect] = off;
+    off += (long long)(x1 - x0) * (z1 - z0);
+    f.nrect++;
+  }
+  f.off[f.nrect] = off;
+}
+
+// False when the grid has no undamped interior with room for the band.
+bool recon_shape(const fdmod_ctx *c, Recon &rg)
+{
+  const Grid &g = c->g;
+  rg.rx0 = g.e;
+  rg.rx1 = g.Nx - g.e;
+  rg.rz0 = c->p.freesurface ? g.e + 1 : g.e;
+  rg.rz1 = g.Nz - g.e;
+  if (rg.rx1 <= rg.rx0 || rg.rz1 <= rg.rz0 || rg.rx0 < g.ixStart || rg.rz0 < g.izStart || rg.rx0 < kN ||
+      rg.rz0 < kN || rg.rx1 + kN > g.Nx || rg.rz1 + kN > g.Nz)
+    return false;
+  const int bx0 = rg.rx0 - kN, bx1 = rg.rx1 + kN, bz0 = rg.rz0 - kN, bz1 = rg.rz1 + kN;
+  const int band[4][4] = {{bx0, bx1, bz0, rg.rz0},
+                          {bx0, bx1, rg.rz1, bz1},
+                          {bx0, rg.rx0, rg.rz0, rg.rz1},
+                          {rg.rx1, bx1, rg.rz0, rg.rz1}};
+  const int frame[4][4] = {{g.ixStart, g.ixEnd, g.izStart, rg.rz0},
+                           {g.ixStart, g.ixEnd, rg.rz1, g.izEnd},
+                           {g.ixStart, rg.rx0, rg.rz0, rg.rz1},
+                           {rg.rx1, g.ixEnd, rg.rz0, rg.rz1}};
+  set_rects(rg.band, band, 4, g.ldz);
+  set_rects(rg.frame, frame, 4, g.ldz);
+  rg.nband = (size_t)rg.band.off[rg.band.nrect];
+  rg.nframe = (size_t)rg.frame.off[rg.frame.nrect];
+  return true;
+}
+
+int recon_geometry(fdmod_ctx *c, Recon &rg)
+{
+  if (!recon_shape(c, rg))
+    return fail(c, FDMOD_EINVAL, "boundary-saving reconstruction: the grid has no undamped interior");
+  return FDMOD_OK;
+}
+
+// R must be undamped for the reversal to be exact: count the points where it is not.
+int check_undamped(fdmod_ctx *c, const Recon &rg)
+{
+  unsigned *cnt = (unsigned *)(c->d_red + kRedBlocks + 4);
+  unsigned h = 0;
+  HIPCALL(c, hipMemsetAsync(cnt, 0, sizeof(unsigned), c->stream));
+  launch_count_damped(c->stream, c->d.a[FDMOD_A_COEF1], c->d.a[FDMOD_A_COEF2], c->g.dt, c->g.ldz, rg.rx0, rg.rx1,
+                      rg.rz0, rg.rz1, cnt);
+  HIPCALL(c, hipMemcpyAsync(&h, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  if (h) return fail(c, FDMOD_EINVAL, "boundary-saving reconstruction: %u damped points inside the interior", h);
+  return FDMOD_OK;
+}
+
+// -coef2 and -src for the reversed steps.
+int recon_negated(fdmod_ctx *c)
+{
+  const size_t n = field_floats(c), ns = (size_t)c->ns * c->p.nt;
+  if (!c->d_coef2n) HIPCALL(c, hipMalloc(&c->d_coef2n, n * sizeof(float)));
+  if (ns && c->d_srcn_n < ns) {
+    dfree(c->d_srcn);
+    c->d_srcn = nullptr;
+    c->d_srcn_n = 0;
+    HIPCALL(c, hipMalloc(&c->d_srcn, ns * sizeof(float)));
+    c->d_srcn_n = ns;
+  }
+  launch_negate(c->stream, c->d.a[FDMOD_A_COEF2], c->d_coef2n, n);
+  if (ns) launch_negate(c->stream, c->d_src, c->d_srcn, ns);
+  return FDMOD_OK;
+}
+
+// Copy nf fields between the grid and a compact frame store.
+void frame_io(fdmod_ctx *c, const FrameIO &shape, float *const *fields, int nf, float *pack, bool save)
+{
+  FrameIO f = shape;
+  f.nf = nf;
+  f.save = save ? 1 : 0;
+  for (int i = 0; i < nf; i++) f.grid[i] = fields[i];
+  f.pack = pack;
+  launch_frame_io(c->stream, f);
+}
+
 // Run the whole adjoint loop with the given data-space sources (no imaging).
 int adjoint_only(fdmod_ctx *c, const DataSrc &ds, double *srcadj)
 {
@@ -291,9 +402,13 @@ int reduce_to_host(fdmod_ctx *c, double *out)
 struct GradRun {
   fdmod_ctx *c = nullptr;
   bool f16 = false, ck16 = false, history = false;  // FP16 strain / FP16 checkpoints
+  bool recon = false;                               // boundary-saving reconstruction
   int ncp = 1, seg = 1;
+  int nanch = 0;                                    // reconstruction: full states kept
+  Recon rg;
+  float *band = nullptr, *frame = nullptr;          // reconstruction: per-step stores
   DataSrc ds;
-  SweepArgs sa;
+  SweepArgs sa, rsa;                                // forward / time-reversed sweeps
   int sgi = 0, k = 0, phase = 0;  // backward cursor: segment, step, 0 = recompute / 1 = adjoint
   ImgBlock blk{};
   bool done = false;
@@ -305,6 +420,10 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   if (!opts) return fail(c, FDMOD_EINVAL, "fdmod_gradient: opts must not be NULL");
   if (opts->storage < FDMOD_STORE_F32 || opts->storage > FDMOD_STORE_F16_ALL)
     return fail(c, FDMOD_EINVAL, "fdmod_gradient: storage must be FDMOD_STORE_F32, _F16 or _F16_ALL");
+  if (opts->history < FDMOD_HIST_CHECKPOINTS || opts->history > FDMOD_HIST_RECONSTRUCT)
+    return fail(c, FDMOD_EINVAL, "fdmod_gradient: history must be FDMOD_HIST_CHECKPOINTS, _FULL or _RECONSTRUCT");
+  if (opts->history == FDMOD_HIST_RECONSTRUCT && opts->storage != FDMOD_STORE_F32)
+    return fail(c, FDMOD_EINVAL, "fdmod_gradient: boundary-saving reconstruction stores FP32 only");
   int rc = check_ready(c, "fdmod_gradient");
   if (rc) return rc;
   const fdmod_params &p = c->p;
@@ -312,7 +431,8 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   r.c = c;
   r.f16 = opts->storage != FDMOD_STORE_F32;
   r.ck16 = opts->storage == FDMOD_STORE_F16_ALL;
-  r.history = opts->history != 0;
+  r.history = opts->history == FDMOD_HIST_FULL;
+  r.recon = opts->history == FDMOD_HIST_RECONSTRUCT;
   if ((rc = workspace(c, true))) return rc;
   if ((rc = upload_data(c, obs_vx, obs_vz, obs_p, r.ds))) return rc;
   // storage: the whole velocity history, or ncp states one every seg steps plus the
@@ -321,7 +441,21 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   ncp = std::min(std::max(ncp, 1), p.nt);
   r.seg = (p.nt + ncp - 1) / ncp;
   r.ncp = (p.nt + r.seg - 1) / r.seg;
-  if (r.history) {
+  if (r.recon) {
+    // [nanch full states][nt bands of n_state fields][nt frames of n_hist fields];
+    // kImgK history slots (slot 0 doubles as the forward's scratch)
+    if ((rc = recon_geometry(c, r.rg)) || (rc = check_undamped(c, r.rg)) || (rc = recon_negated(c))) return rc;
+    r.nanch = opts->ncheckpoints > 0 ? std::min(opts->ncheckpoints, p.nt) : 0;
+    r.seg = r.nanch > 0 ? (p.nt + r.nanch) / (r.nanch + 1) : p.nt + 1;
+    r.nanch = r.nanch > 0 ? (p.nt - 1) / r.seg : 0;  // states at k = seg, 2 seg, ... < nt
+    const int ns = n_state(p.form), nh = n_hist(p.form);
+    const size_t st = (size_t)r.nanch * ns * field_floats(c);
+    const size_t fl = st + (size_t)p.nt * (ns * r.rg.nband + nh * r.rg.nframe);
+    if ((rc = reserve(c, &c->ckpt.data, c->ckpt.cap_data, std::max<size_t>(fl, 1) * sizeof(float)))) return rc;
+    if ((rc = store_reserve(c, c->hist, kImgK, nh, false))) return rc;
+    r.band = (float *)c->ckpt.data + st;
+    r.frame = r.band + (size_t)p.nt * ns * r.rg.nband;
+  } else if (r.history) {
     r.seg = p.nt;
     r.ncp = 1;
     if ((rc = store_reserve(c, c->hist, p.nt, n_hist(p.form), r.f16))) return rc;
@@ -331,7 +465,91 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   }
   if ((rc = fdmod_reset(c))) return rc;
   r.sa = sweep_args(c);
+  r.rsa = r.sa;
+  r.rsa.g.dt = -r.sa.g.dt;
+  r.rsa.coef2 = c->d_coef2n;
+  r.rsa.rev = 1;
   return zero_adjoint(c);
+}
+
+float *const *state_fields(fdmod_ctx *c, float **f)
+{
+  for (int i = 0; i < n_state(c->p.form); i++) f[i] = c->d.a[kStateIds[i]];
+  return f;
+}
+
+float *const *hist_fields(fdmod_ctx *c, size_t slot, float **f)
+{
+  const int nh = n_hist(c->p.form);
+  for (int i = 0; i < nh; i++) f[i] = (float *)c->hist.data + (slot * nh + i) * field_floats(c);
+  return f;
+}
+
+// Reconstruction, forward step k: the loop iteration with the frame's history stored
+// into slot 0 (not inside R), then the frame history and the band state saved.
+int recon_forward(GradRun &r, int k, int it)
+{
+  fdmod_ctx *c = r.c;
+  const int ns = n_state(c->p.form), nh = n_hist(c->p.form);
+  int rc;
+  if (r.nanch > 0 && k > 0 && k % r.seg == 0 && (rc = state_io(c, k / r.seg - 1, false, true))) return rc;
+  SweepArgs sh = with_hist(c, r.sa, 0, false);
+  sh.hskip[0] = r.rg.rx0;
+  sh.hskip[1] = r.rg.rx1;
+  sh.hskip[2] = r.rg.rz0;
+  sh.hskip[3] = r.rg.rz1;
+  if ((rc = step(c, it, sh, true))) return rc;
+  float *f[kMaxState];
+  frame_io(c, r.rg.frame, hist_fields(c, 0, f), nh, r.frame + (size_t)k * nh * r.rg.nframe, true);
+  frame_io(c, r.rg.band, state_fields(c, f), ns, r.band + (size_t)k * ns * r.rg.nband, true);
+  return FDMOD_OK;
+}
+
+// Reconstruction, backward step k: undo loop iteration k, from (v_{k+1/2},
+// sigma_{k+1}) to (v_{k-1/2}, sigma_k), storing its history into slot `slot`.
+// The reversed stress sweep recomputes the strain of v_{k+1/2}; the tensor form's
+// history takes the sweep's input t, the forward's t before the stress source.
+int recon_backward(GradRun &r, int k, int slot)
+{
+  fdmod_ctx *c = r.c;
+  const fdmod_params &p = c->p;
+  hipStream_t s = c->stream;
+  const int ns = n_state(p.form), nh = n_hist(p.form);
+  const int it = p.mode == 1 ? k : p.nt - 1 - k;
+  const SweepArgs rs = with_hist(c, r.rsa, slot, false);
+  SourceArgs src = source_args(c, it);
+  src.src = c->d_srcn;
+  if (c->ns > 0 && c->stype == 1) {
+    src.fa = rs.sxx;
+    src.fb = rs.szz;
+    src.fc = p.form == FDMOD_FORM_TENSOR ? rs.txx : nullptr;
+    src.fd = p.form == FDMOD_FORM_TENSOR ? rs.tzz : nullptr;
+    launch_source(s, src);
+  }
+  launch_ssweep(s, p.form, rs);
+  float *f[kMaxState];
+  state_fields(c, f);
+  if (k > 0) {
+    float *band = r.band + (size_t)(k - 1) * ns * r.rg.nband;  // saved by forward step k - 1
+    frame_io(c, r.rg.band, f + 2, ns - 2, band + 2 * r.rg.nband, false);  // stresses at step k
+    if (c->ns > 0 && c->stype != 1) {
+      src.fa = c->stype == 6 ? rs.vz : rs.vx;
+      src.fb = (c->stype == 5 || c->stype == 6) ? nullptr : rs.vz;
+      src.fc = src.fd = nullptr;
+      launch_source(s, src);
+    }
+    launch_vsweep(s, p.form, rs);
+    frame_io(c, r.rg.band, f, 2, band, false);  // velocities at k - 1/2
+  } else if (p.form == FDMOD_FORM_COORD) {
+    // the velocity-sweep history of step 0 is that of the zero initial stress
+    float *h[kMaxState];
+    hist_fields(c, slot, h);
+    for (int i = 4; i < 8; i++) HIPCALL(c, hipMemsetAsync(h[i], 0, field_floats(c) * sizeof(float), s));
+  }
+  float *h[kMaxState];
+  frame_io(c, r.rg.frame, hist_fields(c, slot, h), nh, r.frame + (size_t)k * nh * r.rg.nframe, false);
+  if (r.nanch > 0 && k > 0 && k % r.seg == 0) return state_io(c, k / r.seg - 1, false, false);
+  return FDMOD_OK;
 }
 
 // Forward step k from rest: checkpoint or history store, then the loop iteration.
@@ -340,8 +558,9 @@ int grad_forward(GradRun &r, int k)
   fdmod_ctx *c = r.c;
   const fdmod_params &p = c->p;
   int rc;
-  if (!r.history && k % r.seg == 0 && (rc = state_io(c, k / r.seg, r.ck16, true))) return rc;
   const int it = p.mode == 1 ? k : p.nt - 1 - k;
+  if (r.recon) return recon_forward(r, k, it);
+  if (!r.history && k % r.seg == 0 && (rc = state_io(c, k / r.seg, r.ck16, true))) return rc;
   if (r.history) return step(c, it, with_hist(c, r.sa, k, r.f16), true);
   return step(c, it, r.sa, true);
 }
@@ -361,6 +580,10 @@ int grad_residual(GradRun &r)
   r.sgi = r.ncp - 1;
   r.phase = r.history ? 1 : 0;
   r.k = r.history ? r.c->p.nt - 1 : r.sgi * r.seg;
+  if (r.recon) {
+    r.phase = 2;
+    r.k = r.c->p.nt - 1;
+  }
   return FDMOD_OK;
 }
 
@@ -373,6 +596,19 @@ int grad_backward_next(GradRun &r)
   const fdmod_params &p = c->p;
   const int k0 = r.sgi * r.seg, k1 = std::min(p.nt, k0 + r.seg);
   int rc;
+  if (r.phase == 2) {  // reconstruction: reversed iteration, then the adjoint step
+    const int j = r.blk.nk;
+    if ((rc = recon_backward(r, r.k, j))) return rc;
+    adj_step(c, r.k, r.ds, j, nullptr);
+    add_to_block(c, r.blk, j, j, false);
+    if (++r.blk.nk == kImgK || r.k == 0) {
+      launch_image(c->stream, adj_args(c, false), r.blk);
+      r.blk = ImgBlock{};
+    }
+    if (r.k > 0) r.k--;
+    else r.done = true;
+    return FDMOD_OK;
+  }
   if (r.phase == 0) {
     if (r.k == k0 && (rc = state_io(c, r.sgi, r.ck16, false))) return rc;
     if ((rc = step(c, p.mode == 1 ? r.k : p.nt - 1 - r.k, with_hist(c, r.sa, r.k - k0, r.f16), false)))
@@ -426,7 +662,12 @@ size_t grad_bytes(const fdmod_ctx *c, const fdmod_grad_opts &o)
   const size_t esz = o.storage == FDMOD_STORE_F32 ? 4 : 2, cksz = o.storage == FDMOD_STORE_F16_ALL ? 2 : 4;
   const int form = c->p.form;
   size_t b = n * (kMaxState * 4 + kNG * 8 + kImgK * 6 * 4);
-  if (o.history) {
+  Recon rg;
+  if (o.history == FDMOD_HIST_RECONSTRUCT && recon_shape(c, rg)) {
+    const size_t nanch = o.ncheckpoints > 0 ? std::min<size_t>(o.ncheckpoints, nt) : 0;
+    b += (kImgK * n_hist(form) + nanch * n_state(form)) * n * 4 +
+         nt * (n_state(form) * rg.nband + n_hist(form) * rg.nframe) * 4 + n * 4;
+  } else if (o.history == FDMOD_HIST_FULL) {
     b += nt * n_hist(form) * n * esz;
   } else {
     int ncp = o.ncheckpoints > 0 ? o.ncheckpoints : (int)ceil(sqrt((double)nt));

@@ -70,6 +70,13 @@ struct SweepArgs {
   float *h32[8];
   _Float16 *h16[8];
   float *hsc[2];
+  // Time-reversed runs (dt and coef2 negated by the caller): rev = 1 makes the
+  // tensor-form history hold the stress sweep's input t, which is the forward's t
+  // before the stress source once the caller has taken the source off.  FP32
+  // history is not stored inside the box hskip = {x0, x1, z0, z1} (columns x0..x1-1,
+  // rows z0..z1-1; an empty box stores everywhere).
+  int rev;
+  int hskip[4];
   // Single-shot loop, Bond and coordinate forms: the stress sweep also adds the stress
   // source (fuse bit 1: stype 1, one source whose 11 x 11 footprint the sweep writes;
   // ftap = its tap table, ordered ox-major) and records the pressure receivers (bit 2:
@@ -149,6 +156,17 @@ struct Pack16 {
   float *inv[8];
 };
 
+// Compact copy of a set of up to 4 rectangles of grid points (rows fastest inside
+// each rectangle) for nf fields: pack[f * npts + i] <-> grid[f][x * ldz + z], where
+// i enumerates the rectangles' points in order and npts = off[nrect].
+struct FrameIO {
+  int nrect, ldz, nf, save;  // save = 1: grid -> pack, 0: pack -> grid
+  int x0[4], z0[4], h[4];    // rectangle r: columns x0 .., rows z0 .. z0 + h - 1
+  long long off[5];          // prefix sums of the rectangles' point counts
+  float *grid[8];
+  float *pack;
+};
+
 // ---- launchers (kernels_setup.hip) ----
 void launch_pad(hipStream_t s, const float *raw, float *dst, int nz, int nx, const Grid &g);
 void launch_meal(hipStream_t s, const Grid &g, float *vp, float *vs, float *rho, float *D, float *Beta,
@@ -197,5 +215,13 @@ void launch_fold_trig(hipStream_t s, const Grid &g, double *G, size_t gstride);
 void launch_rescale_frozen(hipStream_t s, const Grid &g, float *vp, float *vs, float *rho, const float *Beta);
 void launch_pack16(hipStream_t s, const Pack16 &p, bool pack);
 void launch_accum_grad(hipStream_t s, const double *src, int nx, int nz, int x0, int nx_full, double *dst);
+
+// ---- launchers (kernels_recon.hip) ----
+void launch_frame_io(hipStream_t s, const FrameIO &a);
+void launch_negate(hipStream_t s, const float *in, float *out, size_t n);
+// Adds to *count the points of columns x0..x1-1, rows z0..z1-1 where coef1 != 1 or
+// coef2 != dt.
+void launch_count_damped(hipStream_t s, const float *coef1, const float *coef2, float dt, int ldz, int x0, int x1,
+                         int z0, int z1, unsigned *count);
 
 }  // namespace fdm

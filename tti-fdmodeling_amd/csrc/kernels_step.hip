@@ -27,7 +27,9 @@ __device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, 
                                            int f0, int grp)
 {
   if (ESTORE == 1) {
-    if (zok)
+    const int iz = (int)(o - (size_t)c * a.g.ldz);
+    const bool skip = c >= a.hskip[0] && c < a.hskip[1] && iz >= a.hskip[2] && iz < a.hskip[3];
+    if (zok && !skip)
 #pragma unroll
       for (int f = 0; f < NF; f++) a.h32[f0 + f][o] = e[f];
     return;
@@ -279,9 +281,9 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
           st_out(a.txx + o, t1);
           st_out(a.tzz + o, t2);
           st_out(a.txz + o, t3);
-          hv[3] = t1;
-          hv[4] = t2;
-          hv[5] = t3;
+          hv[3] = a.rev ? op.s1 : t1;
+          hv[4] = a.rev ? op.s2 : t2;
+          hv[5] = a.rev ? op.s3 : t3;
           nxx = K2 * t1 + S2 * t2 + S2T * t3;
           nzz = S2 * t1 + K2 * t2 - S2T * t3;
           nxz = -0.5f * S2T * (t1 - t2) + (K2 - S2) * t3;

@@ -9,9 +9,16 @@ import ctypes as ct
 
 import numpy as np
 
-from ._lib import AID, FORMS, STORAGE, FdmodError, GradOpts, Params, SNAPSHOT_FN, _f32, _ptr, load
+from ._lib import AID, FORMS, HISTORY, STORAGE, FdmodError, GradOpts, Params, SNAPSHOT_FN, _f32, _ptr, load
 
 __all__ = ["Modeling", "Batch", "FdmodError", "FORMS", "load"]
+
+
+def _history_mode(history):
+    """fdmod_grad_opts.history from the Python spelling."""
+    if isinstance(history, str):
+        return HISTORY[history]
+    return int(bool(history))
 
 
 class Modeling:
@@ -152,12 +159,15 @@ class Modeling:
     def gradient(self, obs_vx=None, obs_vz=None, obs_p=None, ncheckpoints=0, storage="f32", history=False):
         """J = 1/2 |d - obs|^2 over the given components and dJ/d(vp, eps, delta, theta[deg])
         as a float64 array (4, nx, nz); runs the forward from rest.
-        storage "f32" or "f16" (stored wavefields); history keeps every step's velocity
-        instead of checkpoints."""
+        storage "f32", "f16" or "f16all" (stored wavefields).  history: False or
+        "checkpoints" (ncheckpoints states, segments recomputed), True or "full" (the
+        strain of every step), "reconstruct" (boundary-saving reconstruction: the loop
+        run backwards over the undamped interior from saved band states; ncheckpoints > 0
+        adds that many restart states)."""
         keep, ptrs = self._series((obs_vx, obs_vz, obs_p))
         J = ct.c_double(0)
         g = np.empty(4 * self.nx * self.nz, dtype=np.float64)
-        opts = GradOpts(int(ncheckpoints), STORAGE[storage], int(bool(history)))
+        opts = GradOpts(int(ncheckpoints), STORAGE[storage], _history_mode(history))
         self._check(self.L.fdmod_gradient_opts(self.c, *ptrs, ct.byref(opts), ct.byref(J), _ptr(g, ct.c_double)))
         return J.value, g.reshape(4, self.nx, self.nz)
 
@@ -284,7 +294,7 @@ class Batch:
             return arr
 
         pv, pz, pp = ptrs(obs_vx), ptrs(obs_vz), ptrs(obs_p)
-        opts = GradOpts(int(ncheckpoints), STORAGE[storage], int(bool(history)))
+        opts = GradOpts(int(ncheckpoints), STORAGE[storage], _history_mode(history))
         J = ct.c_double(0)
         if x0 is not None:
             x0a = np.ascontiguousarray(x0, dtype=np.int32)
