@@ -197,10 +197,18 @@ int fdmod_gradient(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, con
  *                            band back after each sweep.  ncheckpoints > 0 also stores that
  *                            many full states and restarts the reversed run from each (0:
  *                            none).  FP32 only (storage must be FDMOD_STORE_F32).
+ *   FDMOD_HIST_RECONSTRUCT_INTERIOR  as _RECONSTRUCT without keeping the absorbing layers'
+ *                            strain (only the band, and the surface row under a free
+ *                            surface, are saved): the gradient leaves out the absorbing
+ *                            layers' imaging, which the transpose of the edge padding adds
+ *                            only to the model's first and last columns, its bottom row
+ *                            and, without a free surface, its top row; every other cell is
+ *                            that of _RECONSTRUCT.
  * fdmod_gradient is {ncheckpoints, FDMOD_STORE_F32, FDMOD_HIST_CHECKPOINTS}. */
 #define FDMOD_HIST_CHECKPOINTS 0
 #define FDMOD_HIST_FULL 1
 #define FDMOD_HIST_RECONSTRUCT 2
+#define FDMOD_HIST_RECONSTRUCT_INTERIOR 3
 #define FDMOD_STORE_F32 0
 #define FDMOD_STORE_F16 1      /* strain FP16, checkpoints FP32 */
 #define FDMOD_STORE_F16_ALL 2  /* strain and checkpoints FP16: restarts from rounded
@@ -209,7 +217,7 @@ int fdmod_gradient(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, con
 typedef struct fdmod_grad_opts {
   int ncheckpoints; /* <= 0: ceil(sqrt(nt))                    */
   int storage;      /* FDMOD_STORE_F32, _F16 or _F16_ALL        */
-  int history;      /* FDMOD_HIST_CHECKPOINTS, _FULL or _RECONSTRUCT */
+  int history;      /* FDMOD_HIST_CHECKPOINTS, _FULL, _RECONSTRUCT or _RECONSTRUCT_INTERIOR */
 } fdmod_grad_opts;
 int fdmod_gradient_opts(fdmod_ctx *ctx, const float *obs_vx, const float *obs_vz, const float *obs_p,
                         const fdmod_grad_opts *opts, double *misfit, double *grad);

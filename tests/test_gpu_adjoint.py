@@ -223,3 +223,22 @@ def test_reconstruction_rejects_fp16_storage():
     g = _mk(Modeling, m, 0, seis=2)
     with pytest.raises(FdmodError, match="FP32"):
         g.gradient(*obs, storage="f16", history="reconstruct")
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("fs", [0, 1])
+def test_gradient_reconstruction_interior(fs, form):
+    # Without the absorbing layers' imaging the gradient changes only where the
+    # transpose of the edge padding adds those layers: the first and last model
+    # columns, the bottom row and, without a free surface, the top row.
+    print("reconstruct_interior", form, fs, flush=True)
+    m, obs = _obs_src(fs, 1, 2, 1, form)
+    g = _mk(Modeling, m, fs, seis=2, form=form)
+    Jc, gc = g.gradient(*obs)
+    J, gi = g.gradient(*obs, history="reconstruct_interior")
+    assert J == Jc
+    inner = (slice(1, -1), slice(0 if fs else 1, -1))
+    errs = [rel_l2(gi[i][inner], gc[i][inner]) for i in range(4)]
+    edge = [rel_l2(gi[i], gc[i]) for i in range(4)]
+    print("  inner", ["%.2e" % e for e in errs], "whole", ["%.2e" % e for e in edge], flush=True)
+    assert max(errs) <= 1e-5, errs

@@ -278,7 +278,13 @@ int state_io(fdmod_ctx *c, size_t slot, bool f16, bool save)
 // back after each reversed sweep.  Outside R the reversed fields are not the
 // forward's, so the history of the frame U \ R (U = the update range) is saved by
 // the forward and written over the history the reversed sweeps store there.
+// Interior variant: U \ R is the absorbing layers (plus the surface row under a free
+// surface), whose imaging adds, through the transpose of the edge padding, only to
+// the model's outermost cells.  It saves the surface row's history alone, stores the
+// reversed history inside R only and images zero elsewhere: the gradient then leaves
+// out the absorbing layers' contributions to those cells.
 struct Recon {
+  bool interior = false;
   int rx0 = 0, rx1 = 0, rz0 = 0, rz1 = 0;
   FrameIO band{}, frame{};  // rectangles only; fields, pack and direction set per use
   size_t nband = 0, nframe = 0;
@@ -303,9 +309,10 @@ void set_rects(FrameIO &f, const int (*r)[4], int n, int ldz)
 }
 
 // False when the grid has no undamped interior with room for the band.
-bool recon_shape(const fdmod_ctx *c, Recon &rg)
+bool recon_shape(const fdmod_ctx *c, Recon &rg, bool interior)
 {
   const Grid &g = c->g;
+  rg.interior = interior;
   rg.rx0 = g.e;
   rg.rx1 = g.Nx - g.e;
   rg.rz0 = c->p.freesurface ? g.e + 1 : g.e;
@@ -322,16 +329,18 @@ bool recon_shape(const fdmod_ctx *c, Recon &rg)
                            {g.ixStart, g.ixEnd, rg.rz1, g.izEnd},
                            {g.ixStart, rg.rx0, rg.rz0, rg.rz1},
                            {rg.rx1, g.ixEnd, rg.rz0, rg.rz1}};
+  const int surface[1][4] = {{rg.rx0, rg.rx1, g.izStart, rg.rz0}};  // empty without a free surface
   set_rects(rg.band, band, 4, g.ldz);
-  set_rects(rg.frame, frame, 4, g.ldz);
+  if (interior) set_rects(rg.frame, surface, 1, g.ldz);
+  else set_rects(rg.frame, frame, 4, g.ldz);
   rg.nband = (size_t)rg.band.off[rg.band.nrect];
   rg.nframe = (size_t)rg.frame.off[rg.frame.nrect];
   return true;
 }
 
-int recon_geometry(fdmod_ctx *c, Recon &rg)
+int recon_geometry(fdmod_ctx *c, Recon &rg, bool interior)
 {
-  if (!recon_shape(c, rg))
+  if (!recon_shape(c, rg, interior))
     return fail(c, FDMOD_EINVAL, "boundary-saving reconstruction: the grid has no undamped interior");
   return FDMOD_OK;
 }
@@ -420,9 +429,10 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   if (!opts) return fail(c, FDMOD_EINVAL, "fdmod_gradient: opts must not be NULL");
   if (opts->storage < FDMOD_STORE_F32 || opts->storage > FDMOD_STORE_F16_ALL)
     return fail(c, FDMOD_EINVAL, "fdmod_gradient: storage must be FDMOD_STORE_F32, _F16 or _F16_ALL");
-  if (opts->history < FDMOD_HIST_CHECKPOINTS || opts->history > FDMOD_HIST_RECONSTRUCT)
-    return fail(c, FDMOD_EINVAL, "fdmod_gradient: history must be FDMOD_HIST_CHECKPOINTS, _FULL or _RECONSTRUCT");
-  if (opts->history == FDMOD_HIST_RECONSTRUCT && opts->storage != FDMOD_STORE_F32)
+  if (opts->history < FDMOD_HIST_CHECKPOINTS || opts->history > FDMOD_HIST_RECONSTRUCT_INTERIOR)
+    return fail(c, FDMOD_EINVAL,
+                "fdmod_gradient: history must be FDMOD_HIST_CHECKPOINTS, _FULL, _RECONSTRUCT or _RECONSTRUCT_INTERIOR");
+  if (opts->history >= FDMOD_HIST_RECONSTRUCT && opts->storage != FDMOD_STORE_F32)
     return fail(c, FDMOD_EINVAL, "fdmod_gradient: boundary-saving reconstruction stores FP32 only");
   int rc = check_ready(c, "fdmod_gradient");
   if (rc) return rc;
@@ -432,7 +442,7 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   r.f16 = opts->storage != FDMOD_STORE_F32;
   r.ck16 = opts->storage == FDMOD_STORE_F16_ALL;
   r.history = opts->history == FDMOD_HIST_FULL;
-  r.recon = opts->history == FDMOD_HIST_RECONSTRUCT;
+  r.recon = opts->history >= FDMOD_HIST_RECONSTRUCT;
   if ((rc = workspace(c, true))) return rc;
   if ((rc = upload_data(c, obs_vx, obs_vz, obs_p, r.ds))) return rc;
   // storage: the whole velocity history, or ncp states one every seg steps plus the
@@ -444,7 +454,9 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   if (r.recon) {
     // [nanch full states][nt bands of n_state fields][nt frames of n_hist fields];
     // kImgK history slots (slot 0 doubles as the forward's scratch)
-    if ((rc = recon_geometry(c, r.rg)) || (rc = check_undamped(c, r.rg)) || (rc = recon_negated(c))) return rc;
+    const bool interior = opts->history == FDMOD_HIST_RECONSTRUCT_INTERIOR;
+    if ((rc = recon_geometry(c, r.rg, interior)) || (rc = check_undamped(c, r.rg)) || (rc = recon_negated(c)))
+      return rc;
     r.nanch = opts->ncheckpoints > 0 ? std::min(opts->ncheckpoints, p.nt) : 0;
     r.seg = r.nanch > 0 ? (p.nt + r.nanch) / (r.nanch + 1) : p.nt + 1;
     r.nanch = r.nanch > 0 ? (p.nt - 1) / r.seg : 0;  // states at k = seg, 2 seg, ... < nt
@@ -453,6 +465,8 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
     const size_t fl = st + (size_t)p.nt * (ns * r.rg.nband + nh * r.rg.nframe);
     if ((rc = reserve(c, &c->ckpt.data, c->ckpt.cap_data, std::max<size_t>(fl, 1) * sizeof(float)))) return rc;
     if ((rc = store_reserve(c, c->hist, kImgK, nh, false))) return rc;
+    if (interior)  // the reversed sweeps store inside R only: the rest images zero
+      HIPCALL(c, hipMemsetAsync(c->hist.data, 0, (size_t)kImgK * nh * field_floats(c) * sizeof(float), c->stream));
     r.band = (float *)c->ckpt.data + st;
     r.frame = r.band + (size_t)p.nt * ns * r.rg.nband;
   } else if (r.history) {
@@ -469,6 +483,13 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   r.rsa.g.dt = -r.sa.g.dt;
   r.rsa.coef2 = c->d_coef2n;
   r.rsa.rev = 1;
+  if (r.recon && r.rg.interior) {
+    r.rsa.hskip[0] = r.rg.rx0;
+    r.rsa.hskip[1] = r.rg.rx1;
+    r.rsa.hskip[2] = r.rg.rz0;
+    r.rsa.hskip[3] = r.rg.rz1;
+    r.rsa.hinside = 1;
+  }
   return zero_adjoint(c);
 }
 
@@ -486,21 +507,33 @@ float *const *hist_fields(fdmod_ctx *c, size_t slot, float **f)
 }
 
 // Reconstruction, forward step k: the loop iteration with the frame's history stored
-// into slot 0 (not inside R), then the frame history and the band state saved.
+// into slot 0 (nothing inside R), then the frame history and the band state saved.
 int recon_forward(GradRun &r, int k, int it)
 {
   fdmod_ctx *c = r.c;
   const int ns = n_state(c->p.form), nh = n_hist(c->p.form);
   int rc;
   if (r.nanch > 0 && k > 0 && k % r.seg == 0 && (rc = state_io(c, k / r.seg - 1, false, true))) return rc;
-  SweepArgs sh = with_hist(c, r.sa, 0, false);
-  sh.hskip[0] = r.rg.rx0;
-  sh.hskip[1] = r.rg.rx1;
-  sh.hskip[2] = r.rg.rz0;
-  sh.hskip[3] = r.rg.rz1;
-  if ((rc = step(c, it, sh, true))) return rc;
   float *f[kMaxState];
-  frame_io(c, r.rg.frame, hist_fields(c, 0, f), nh, r.frame + (size_t)k * nh * r.rg.nframe, true);
+  if (r.rg.nframe == 0) {  // interior variant without a free surface: no history to keep
+    if ((rc = step(c, it, r.sa, true))) return rc;
+  } else {
+    SweepArgs sh = with_hist(c, r.sa, 0, false);
+    if (r.rg.interior) {  // the surface row
+      sh.hskip[0] = r.rg.rx0;
+      sh.hskip[1] = r.rg.rx1;
+      sh.hskip[2] = c->g.izStart;
+      sh.hskip[3] = r.rg.rz0;
+      sh.hinside = 1;
+    } else {
+      sh.hskip[0] = r.rg.rx0;
+      sh.hskip[1] = r.rg.rx1;
+      sh.hskip[2] = r.rg.rz0;
+      sh.hskip[3] = r.rg.rz1;
+    }
+    if ((rc = step(c, it, sh, true))) return rc;
+    frame_io(c, r.rg.frame, hist_fields(c, 0, f), nh, r.frame + (size_t)k * nh * r.rg.nframe, true);
+  }
   frame_io(c, r.rg.band, state_fields(c, f), ns, r.band + (size_t)k * ns * r.rg.nband, true);
   return FDMOD_OK;
 }
@@ -663,7 +696,7 @@ size_t grad_bytes(const fdmod_ctx *c, const fdmod_grad_opts &o)
   const int form = c->p.form;
   size_t b = n * (kMaxState * 4 + kNG * 8 + kImgK * 6 * 4);
   Recon rg;
-  if (o.history == FDMOD_HIST_RECONSTRUCT && recon_shape(c, rg)) {
+  if (o.history >= FDMOD_HIST_RECONSTRUCT && recon_shape(c, rg, o.history == FDMOD_HIST_RECONSTRUCT_INTERIOR)) {
     const size_t nanch = o.ncheckpoints > 0 ? std::min<size_t>(o.ncheckpoints, nt) : 0;
     b += (kImgK * n_hist(form) + nanch * n_state(form)) * n * 4 +
          nt * (n_state(form) * rg.nband + n_hist(form) * rg.nframe) * 4 + n * 4;

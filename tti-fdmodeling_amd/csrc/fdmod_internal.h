@@ -74,9 +74,10 @@ struct SweepArgs {
   // tensor-form history hold the stress sweep's input t, which is the forward's t
   // before the stress source once the caller has taken the source off.  FP32
   // history is not stored inside the box hskip = {x0, x1, z0, z1} (columns x0..x1-1,
-  // rows z0..z1-1; an empty box stores everywhere).
+  // rows z0..z1-1; an empty box stores everywhere), or with hinside = 1 only inside it.
   int rev;
   int hskip[4];
+  int hinside;
   // Single-shot loop, Bond and coordinate forms: the stress sweep also adds the stress
   // source (fuse bit 1: stype 1, one source whose 11 x 11 footprint the sweep writes;
   // ftap = its tap table, ordered ox-major) and records the pressure receivers (bit 2:

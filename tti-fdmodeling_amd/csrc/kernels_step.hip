@@ -28,8 +28,8 @@ __device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, 
 {
   if (ESTORE == 1) {
     const int iz = (int)(o - (size_t)c * a.g.ldz);
-    const bool skip = c >= a.hskip[0] && c < a.hskip[1] && iz >= a.hskip[2] && iz < a.hskip[3];
-    if (zok && !skip)
+    const bool in = c >= a.hskip[0] && c < a.hskip[1] && iz >= a.hskip[2] && iz < a.hskip[3];
+    if (zok && in == (a.hinside != 0))
 #pragma unroll
       for (int f = 0; f < NF; f++) a.h32[f0 + f][o] = e[f];
     return;

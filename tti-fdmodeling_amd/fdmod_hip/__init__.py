@@ -163,7 +163,8 @@ class Modeling:
         "checkpoints" (ncheckpoints states, segments recomputed), True or "full" (the
         strain of every step), "reconstruct" (boundary-saving reconstruction: the loop
         run backwards over the undamped interior from saved band states; ncheckpoints > 0
-        adds that many restart states)."""
+        adds that many restart states), "reconstruct_interior" (the same without the
+        absorbing layers' imaging, which only the model's edge cells receive)."""
         keep, ptrs = self._series((obs_vx, obs_vz, obs_p))
         J = ct.c_double(0)
         g = np.empty(4 * self.nx * self.nz, dtype=np.float64)

@@ -38,7 +38,7 @@ class GradOpts(ct.Structure):
 
 
 STORAGE = {"f32": 0, "f16": 1, "f16all": 2}
-HISTORY = {"checkpoints": 0, "full": 1, "reconstruct": 2}  # fdmod_grad_opts.history
+HISTORY = {"checkpoints": 0, "full": 1, "reconstruct": 2, "reconstruct_interior": 3}  # fdmod_grad_opts.history
 
 SNAPSHOT_FN = ct.CFUNCTYPE(ct.c_int, ct.c_void_p, ct.c_int, ct.c_char_p, _fp, ct.c_int, ct.c_int)
 
