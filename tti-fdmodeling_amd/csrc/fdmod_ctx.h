@@ -59,6 +59,7 @@ struct fdmod_ctx {
   double *d_red = nullptr;     // reduction partials (kRedBlocks) and scalars
   double *d_srcadj = nullptr;  // adjoint of the source time functions [ns][nt]
   double *d_grad = nullptr;    // gradient [4][nx][nz]
+  int adj_segS = 0, adj_segV = 0;  // adjoint sweeps' segment lengths (0: not chosen yet)
   float *d_coef2n = nullptr;   // -coef2 (Nx*ldz) and -src ([ns][nt]) for time-reversed steps
   float *d_srcn = nullptr;
   size_t d_srcn_n = 0;

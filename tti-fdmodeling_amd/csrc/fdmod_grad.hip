@@ -32,6 +32,8 @@ namespace {
 constexpr int kRedBlocks = 256;  // partial sums of the FP64 reductions
 constexpr int kMaxState = 8;     // vx, vz, sxx, szz, sxz (+ txx, tzz, txz for the tensor form)
 constexpr int kNG = 8;           // imaging sums
+constexpr int kAdjSegS = 44, kAdjSegV = 44;  // longest adjoint segments (columns)
+constexpr int kAdjMinTiles = 1450;            // blocks per adjoint sweep below which they shorten
 
 size_t field_floats(const fdmod_ctx *c) { return (size_t)c->g.Nx * c->g.ldz; }
 
@@ -152,10 +154,41 @@ void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
   for (int q = 0; q < 2; q++) b.hsc[j][q] = f16 ? c->hist.scale + (slot * 2 + q) * ns : nullptr;
 }
 
+// Segment lengths of the two adjoint sweeps, chosen once per context.  The adjoint
+// sweeps stage 2-6 fields computed from up to 10 loaded operands, so a segment's
+// halo columns cost them more than they cost the forward sweeps: they keep longer
+// segments (fewer, longer blocks) down to a smaller block count.  On 1126-deep crops
+// 1500, 2000 and 3000 columns wide the fastest pairs were 22/22, 22/22 and 33/33
+// columns, 3-7% faster backward passes than 11-column segments; the longest segment
+// giving at least kAdjMinTiles blocks picks each of them.
+void adj_segments(fdmod_ctx *c)
+{
+  if (c->adj_segS > 0) return;
+  const Grid &g = c->g;
+  auto pick = [&](int longest, int min_tiles) {
+    int L = longest;
+    while (L > kFlagW && g.tilesZ * ((g.ixEnd - g.ixStart + L - 1) / L) < min_tiles) L -= kFlagW;
+    return L;
+  };
+  c->adj_segS = pick(kAdjSegS, kAdjMinTiles);
+  c->adj_segV = pick(kAdjSegV, kAdjMinTiles);
+  if (const char *e = getenv("FDMOD_ADJ_SEGS")) {  // tile-shape overrides (tuning experiments)
+    const int v = atoi(e);
+    if (v >= 1 && v <= 4096) c->adj_segS = v;
+  }
+  if (const char *e = getenv("FDMOD_ADJ_SEGV")) {
+    const int v = atoi(e);
+    if (v >= 1 && v <= 4096) c->adj_segV = v;
+  }
+}
+
 AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
 {
   AdjArgs a{};
+  adj_segments(c);
   a.g = c->g;
+  a.g.segLV = c->adj_segS;  // the stress adjoint is tiled like the velocity sweep
+  a.g.segLS = c->adj_segV;  // and the velocity adjoint like the stress sweep
   a.izU = c->g.izStart;
   if (image_rows && c->p.freesurface) {
     a.g.izStart -= kN;
