@@ -163,6 +163,15 @@ def test_torch_autograd_function():
     with torch.no_grad():
         J2 = misfit(ts["vp"], ts["eps"], ts["delta"], ts["theta"], g, m["vs"], m["rho"], {"p": obs[2]})
     assert abs(J2.item() - J_ref) <= 1e-6 * J_ref
+    # the storage and history options reach the library
+    for k in ts.values():
+        k.grad = None
+    J3 = misfit(ts["vp"], ts["eps"], ts["delta"], ts["theta"], g, m["vs"], m["rho"], {"p": obs[2]},
+                storage="f16", history=True)
+    J3.backward()
+    J_f16, grad_f16 = g.gradient(*obs, storage="f16", history=True)
+    assert J3.item() == J_f16
+    assert rel_l2(ts["vp"].grad.double().cpu().numpy(), grad_f16[0]) <= 1e-6
 
 
 def test_dot_product_at_gradient_workload_size():

@@ -80,7 +80,8 @@ def test_autotuner_and_tile_shapes_keep_results():
             assert np.array_equal(b.traces(i, 4), ref[i]), (lv, ls, i)
 
 
-@pytest.mark.parametrize("storage,history", [("f32", True), ("f16", False)])
+@pytest.mark.parametrize("storage,history", [("f32", True), ("f16", False), ("f32", "reconstruct"),
+                                             ("f32", "reconstruct_interior")])
 def test_batch_gradient_equals_single_shot_gradients(storage, history):
     shots = _shots()
     b = _batch("bond", shots)

@@ -25,11 +25,11 @@ class TTIMisfit(torch.autograd.Function):
     not in the misfit).  Returns J as a float64 0-d tensor on vp's device."""
 
     @staticmethod
-    def forward(ctx, vp, eps, delta, theta, modeling, vs, rho, obs, ncheckpoints=0):
+    def forward(ctx, vp, eps, delta, theta, modeling, vs, rho, obs, ncheckpoints=0, storage="f32", history=False):
         modeling.update_model(_host(vp), _host(vs), _host(rho), _host(eps), _host(delta), _host(theta))
         obs_h = {k: _host(v) for k, v in obs.items()}
         if any(ctx.needs_input_grad[:4]):
-            J, g = modeling.gradient(obs_h.get("vx"), obs_h.get("vz"), obs_h.get("p"), ncheckpoints)
+            J, g = modeling.gradient(obs_h.get("vx"), obs_h.get("vz"), obs_h.get("p"), ncheckpoints, storage, history)
             ctx.save_for_backward(torch.from_numpy(g))
         else:
             modeling.run()
@@ -49,9 +49,10 @@ class TTIMisfit(torch.autograd.Function):
         grads = []
         for i, (dt, dev) in enumerate(ctx.meta):
             grads.append((s * g[i]).to(dt).to(dev) if ctx.needs_input_grad[i] else None)
-        return (*grads, None, None, None, None, None)
+        return (*grads, None, None, None, None, None, None, None)
 
 
-def misfit(vp, eps, delta, theta, modeling, vs, rho, obs, ncheckpoints=0):
-    """Differentiable L2 trace misfit of one shot; see TTIMisfit."""
-    return TTIMisfit.apply(vp, eps, delta, theta, modeling, vs, rho, obs, ncheckpoints)
+def misfit(vp, eps, delta, theta, modeling, vs, rho, obs, ncheckpoints=0, storage="f32", history=False):
+    """Differentiable L2 trace misfit of one shot; see TTIMisfit.  ncheckpoints, storage and
+    history select how the gradient keeps the forward wavefield (Modeling.gradient)."""
+    return TTIMisfit.apply(vp, eps, delta, theta, modeling, vs, rho, obs, ncheckpoints, storage, history)
