@@ -47,10 +47,17 @@ def _batch(form, shots):
     return b
 
 
+# shot groups (run one after another inside each captured chunk): all shots in one
+# group, every shot its own group, and two groups of two (grids of 9728..14592 points)
+@pytest.mark.parametrize("group_pts", [None, "1", "30000"])
 @pytest.mark.parametrize("form", ["bond", "tensor", "coord"])
-def test_batch_equals_single_shots(form):
+def test_batch_equals_single_shots(form, group_pts, monkeypatch):
     shots = _shots()
-    print("batch", form, flush=True)
+    print("batch", form, group_pts, flush=True)
+    if group_pts:
+        monkeypatch.setenv("FDMOD_BATCH_PTS", group_pts)
+    else:
+        monkeypatch.delenv("FDMOD_BATCH_PTS", raising=False)
     b = _batch(form, shots)
     b.run(0, 70)   # two captured chunks + eager remainder, then a second call
     b.run(70, 80)

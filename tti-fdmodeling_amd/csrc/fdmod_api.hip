@@ -743,6 +743,10 @@ extern "C" int fdmod_batch_create(const fdmod_params *p, int nshots, const int *
     fdmod_batch_destroy(b);
     return FDMOD_EDEVICE;
   }
+  if (const char *gc = getenv("FDMOD_BATCH_PTS")) {  // shot-group size override (tuning experiments)
+    const long long v = atoll(gc);
+    if (v >= 1) b->group_pts = (size_t)v;
+  }
   for (int s = 0; s < nshots; s++) {
     fdmod_params q = *p;
     q.nx = nx[s];
@@ -817,15 +821,30 @@ static SourceArgs shot_source(fdmod_ctx *c, const SweepArgs &sa, bool stress_pha
   return s;
 }
 
-static int batch_step(fdmod_batch *b, int off)
+// Loop iteration (step counter + off) of shot group g.
+static int batch_step(fdmod_batch *b, int g, int off)
 {
-  const int n = (int)b->shots.size();
-  launch_vsweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startV, n, b->totalV);
-  if (b->srcBlocksV) launch_source_batch(b->stream, b->d_srcV, n, b->srcBlocksV, b->d_step, off, b->p.nt, b->p.mode);
-  launch_ssweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startS, n, b->totalS);
-  if (b->srcBlocksS) launch_source_batch(b->stream, b->d_srcS, n, b->srcBlocksS, b->d_step, off, b->p.nt, b->p.mode);
-  if (b->maxnr) launch_receivers_batch(b->stream, b->d_rec, n, b->maxnr, b->d_step, off, b->p.nt, b->p.mode);
+  const int s0 = b->grp[g], n = b->grp[g + 1] - s0;
+  launch_vsweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startV + s0 + g, n, b->totV[g]);
+  if (b->srcBlocksV)
+    launch_source_batch(b->stream, b->d_srcV + s0, n, b->srcBlocksV, b->d_step, off, b->p.nt, b->p.mode);
+  launch_ssweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startS + s0 + g, n, b->totS[g]);
+  if (b->srcBlocksS)
+    launch_source_batch(b->stream, b->d_srcS + s0, n, b->srcBlocksS, b->d_step, off, b->p.nt, b->p.mode);
+  if (b->maxnr) launch_receivers_batch(b->stream, b->d_rec + s0, n, b->maxnr, b->d_step, off, b->p.nt, b->p.mode);
   return FDMOD_OK;
+}
+
+// count iterations of every shot, group after group, then the step counter advances.
+// Shots are independent, so the order changes no result, only the cache's reuse: a
+// group's fields are about those of one full-size shot, and its consecutive sweeps
+// find part of each other's fields still in the device's last-level cache, which a
+// sweep over every shot's fields would have evicted.
+static void batch_steps(fdmod_batch *b, int count)
+{
+  for (size_t g = 0; g + 1 < b->grp.size(); g++)
+    for (int j = 0; j < count; j++) batch_step(b, (int)g, j);
+  launch_advance(b->stream, b->d_step, count);
 }
 
 static int batch_prepare(fdmod_batch *b, bool with_graph)
@@ -835,8 +854,23 @@ static int batch_prepare(fdmod_batch *b, bool with_graph)
   std::vector<SweepArgs> sw(n);
   std::vector<SourceArgs> sv(n), ss(n);
   std::vector<RecvArgs> rv(n);
-  std::vector<int> startV(n + 1, 0), startS(n + 1, 0);
   b->srcBlocksV = b->srcBlocksS = b->maxnr = 0;
+  // shot groups of up to group_pts padded points (at least one shot each)
+  b->grp.assign(1, 0);
+  size_t pts = 0;
+  for (int s = 0; s < n; s++) {
+    const size_t w = (size_t)b->shots[s]->g.Nx * b->shots[s]->g.ldz;
+    if (s > b->grp.back() && pts + w > b->group_pts) {
+      b->grp.push_back(s);
+      pts = 0;
+    }
+    pts += w;
+  }
+  b->grp.push_back(n);
+  const int ng = (int)b->grp.size() - 1;
+  std::vector<int> startV(n + ng, 0), startS(n + ng, 0);
+  b->totV.assign(ng, 0);
+  b->totS.assign(ng, 0);
   for (int s = 0; s < n; s++) {
     fdmod_ctx *c = b->shots[s];
     if (!c->model_set) return bfail(b, FDMOD_ESTATE, "shot %d has no model", s);
@@ -844,8 +878,12 @@ static int batch_prepare(fdmod_batch *b, bool with_graph)
     c->g.segLV = b->segLV;
     c->g.segLS = b->segLS;
     sw[s] = sweep_args(c);
-    startV[s + 1] = startV[s] + sweep_tiles(c->g, b->segLV);
-    startS[s + 1] = startS[s] + sweep_tiles(c->g, b->segLS);
+    const int g = (int)(std::upper_bound(b->grp.begin(), b->grp.end(), s) - b->grp.begin()) - 1;
+    const int k = s + g;  // this shot's entry in its group's prefix array
+    startV[k + 1] = startV[k] + sweep_tiles(c->g, b->segLV);
+    startS[k + 1] = startS[k] + sweep_tiles(c->g, b->segLS);
+    b->totV[g] = startV[k + 1];
+    b->totS[g] = startS[k + 1];
     sv[s] = shot_source(c, sw[s], false);
     ss[s] = shot_source(c, sw[s], true);
     if (sv[s].ns) b->srcBlocksV = std::max(b->srcBlocksV, sv[s].sequential ? 1 : sv[s].ns);
@@ -864,25 +902,27 @@ static int batch_prepare(fdmod_batch *b, bool with_graph)
       r.mt = 1;
     }
   }
-  b->totalV = startV[n];
-  b->totalS = startS[n];
+  b->totalV = b->totalS = 0;
+  for (int g = 0; g < ng; g++) {
+    b->totalV += b->totV[g];
+    b->totalS += b->totS[g];
+  }
   BHIP(b, hipMalloc(&b->d_sweep, n * sizeof(SweepArgs)));
-  BHIP(b, hipMalloc(&b->d_startV, (n + 1) * sizeof(int)));
-  BHIP(b, hipMalloc(&b->d_startS, (n + 1) * sizeof(int)));
+  BHIP(b, hipMalloc(&b->d_startV, (n + ng) * sizeof(int)));
+  BHIP(b, hipMalloc(&b->d_startS, (n + ng) * sizeof(int)));
   BHIP(b, hipMalloc(&b->d_srcV, n * sizeof(SourceArgs)));
   BHIP(b, hipMalloc(&b->d_srcS, n * sizeof(SourceArgs)));
   BHIP(b, hipMalloc(&b->d_rec, n * sizeof(RecvArgs)));
   BHIP(b, hipMemcpy(b->d_sweep, sw.data(), n * sizeof(SweepArgs), hipMemcpyHostToDevice));
-  BHIP(b, hipMemcpy(b->d_startV, startV.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
-  BHIP(b, hipMemcpy(b->d_startS, startS.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->d_startV, startV.data(), (n + ng) * sizeof(int), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->d_startS, startS.data(), (n + ng) * sizeof(int), hipMemcpyHostToDevice));
   BHIP(b, hipMemcpy(b->d_srcV, sv.data(), n * sizeof(SourceArgs), hipMemcpyHostToDevice));
   BHIP(b, hipMemcpy(b->d_srcS, ss.data(), n * sizeof(SourceArgs), hipMemcpyHostToDevice));
   BHIP(b, hipMemcpy(b->d_rec, rv.data(), n * sizeof(RecvArgs), hipMemcpyHostToDevice));
   for (fdmod_ctx *c : b->shots) BHIP(b, hipStreamSynchronize(c->stream));
   if (with_graph) {
     BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < b->chunk; j++) batch_step(b, j);
-    launch_advance(b->stream, b->d_step, b->chunk);
+    batch_steps(b, b->chunk);
     hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
     if (e != hipSuccess) return bfail(b, FDMOD_EDEVICE, "graph capture: %s", hipGetErrorString(e));
     BHIP(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
@@ -893,8 +933,7 @@ static int batch_prepare(fdmod_batch *b, bool with_graph)
 
 static int batch_run_eager(fdmod_batch *b, int count)
 {
-  for (int j = 0; j < count; j++) batch_step(b, j);
-  launch_advance(b->stream, b->d_step, count);
+  batch_steps(b, count);
   return FDMOD_OK;
 }
 
@@ -952,21 +991,21 @@ extern "C" int fdmod_batch_tune(fdmod_batch *b, int reps, int *segLV, int *segLS
   if (!b || reps < 1) return FDMOD_EINVAL;
   static const int candV[] = {22, 33, 44, 55, 66, 88};
   static const int candS[] = {11, 22, 33, 44};
-  const int n = (int)b->shots.size();
   auto time_one = [&](bool vsweep, int L, double &ms_out) -> int {
     if (vsweep) b->segLV = L;
     else b->segLS = L;
     int rc = batch_prepare(b, false);
     if (rc) return rc;
-    for (int w = 0; w < 2; w++) {  // warm-up launches
-      if (vsweep) launch_vsweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startV, n, b->totalV);
-      else launch_ssweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startS, n, b->totalS);
-    }
+    auto sweep = [&]() {  // one sweep of every group, as the loop launches it
+      for (size_t g = 0; g + 1 < b->grp.size(); g++) {
+        const int s0 = b->grp[g], ns = b->grp[g + 1] - s0;
+        if (vsweep) launch_vsweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startV + s0 + g, ns, b->totV[g]);
+        else launch_ssweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startS + s0 + g, ns, b->totS[g]);
+      }
+    };
+    for (int w = 0; w < 2; w++) sweep();  // warm-up launches
     BHIP(b, hipEventRecord(b->ev0, b->stream));
-    for (int r = 0; r < reps; r++) {
-      if (vsweep) launch_vsweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startV, n, b->totalV);
-      else launch_ssweep_batch(b->stream, b->p.form, b->d_sweep, b->d_startS, n, b->totalS);
-    }
+    for (int r = 0; r < reps; r++) sweep();
     BHIP(b, hipEventRecord(b->ev1, b->stream));
     BHIP(b, hipEventSynchronize(b->ev1));
     float ms = 0.0f;

@@ -98,6 +98,11 @@ struct fdmod_batch {
   bool prepared = false;
   fdm::SweepArgs *d_sweep = nullptr;
   int *d_startV = nullptr, *d_startS = nullptr, totalV = 0, totalS = 0;
+  // Shot groups run one after another inside each captured chunk of steps: group g is
+  // shots grp[g] .. grp[g+1]-1, its tile prefix sums start at d_startV/S + grp[g] + g,
+  // its tile totals are totV/totS[g].  Groups hold up to group_pts padded points.
+  std::vector<int> grp, totV, totS;
+  size_t group_pts = fdm::kBatchGroupPts;
   fdm::SourceArgs *d_srcV = nullptr, *d_srcS = nullptr;
   int srcBlocksV = 0, srcBlocksS = 0;
   fdm::RecvArgs *d_rec = nullptr;

@@ -31,6 +31,10 @@ constexpr int kFlagW = 11;         // columns per fine flag tile
 constexpr int kDefaultSegLV = 44;  // measured best on a 1256 x 8002 grid
 constexpr int kDefaultSegLS = 11;
 constexpr int kMinTiles = 3000;    // blocks per sweep below which segments shorten
+// Padded grid points per shot group of a batched loop: about one full-size shot
+// (10240 columns of 1280 rows); measured on 64 crops 1126 deep, groups of 9-10k
+// columns ran 8-10% faster than one group (48.3 vs 44.5 Gpts/s).
+constexpr size_t kBatchGroupPts = (size_t)10240 * 1280;
 
 // Device-side geometry of a padded grid.
 struct Grid {
