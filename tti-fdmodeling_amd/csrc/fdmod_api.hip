@@ -117,7 +117,7 @@ extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
   g.segLV = kDefaultSegLV;
   while (g.segLV > kFlagW && g.tilesZ * ((g.ixEnd - g.ixStart + g.segLV - 1) / g.segLV) < kMinTiles)
     g.segLV -= kFlagW;
-  g.segLS = kDefaultSegLS;
+  g.segLS = default_segLS(p->form, g.ixEnd - g.ixStart);
   if (const char *sl = getenv("FDMOD_SEGLV")) {  // tile-shape overrides (tuning experiments)
     int v = atoi(sl);
     if (v >= 1 && v <= 4096) g.segLV = v;
@@ -737,6 +737,7 @@ extern "C" int fdmod_batch_create(const fdmod_params *p, int nshots, const int *
   *out = nullptr;
   fdmod_batch *b = new fdmod_batch();
   b->p = *p;
+  b->segLS = default_segLS(p->form, kWideCols);  // the batch launches cover many shots' columns
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
       hipMalloc(&b->d_step, sizeof(int)) != hipSuccess) {
@@ -990,7 +991,7 @@ extern "C" int fdmod_batch_tune(fdmod_batch *b, int reps, int *segLV, int *segLS
 {
   if (!b || reps < 1) return FDMOD_EINVAL;
   static const int candV[] = {22, 33, 44, 55, 66, 88};
-  static const int candS[] = {11, 22, 33, 44};
+  static const int candS[] = {4, 5, 8, 11, 22};
   auto time_one = [&](bool vsweep, int L, double &ms_out) -> int {
     if (vsweep) b->segLV = L;
     else b->segLS = L;
