@@ -95,6 +95,22 @@ def time_form(form, cfg, m, src, steps, warmup):
     return g, pts, wall, dev_s
 
 
+def host_cpu():
+    """CPU model, logical CPUs and sockets of this host (from /proc/cpuinfo)."""
+    model, sockets = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                if k.strip() == "model name":
+                    model = v.strip()
+                elif k.strip() == "physical id":
+                    sockets.add(v.strip())
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "sockets": max(len(sockets), 1)}
+
+
 def cpu_baseline(cfg, m, src, form, steps, g_fields):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -106,7 +122,7 @@ def cpu_baseline(cfg, m, src, form, steps, g_fields):
     pts = (o.ixEnd - o.ixStart) * (o.izEnd - o.izStart)
     res = {"value": round(pts * steps / sec / 1e9, 4), "unit": "Gpts/s", "cores": cores, "kind": "port",
            "sample": f"first {steps} time steps of the same {form} shot (loop time only, omp_get_wtime)",
-           "seconds": round(sec, 3)}
+           "seconds": round(sec, 3), "host": host_cpu()}
     same = True
     worst = 0.0
     for name, ga in g_fields.items():
@@ -436,17 +452,19 @@ def main():
         parity = {args.form: parity}
         if args.form == "bond":
             # the Bond free-surface fix-up makes this marine model NaN (water at theta = 0,
-            # as in the reference); check a form whose wavefield stays finite as well
+            # as in the reference); check the other two forms, whose wavefields stay finite
             n2 = max(1, min(args.cpu_steps, 400))
-            log(f"bench: coordinate-form bitwise check, {n2} steps")
-            gc = make(__import__("fdmod_hip").Modeling, "coord", cfg, m, src)
-            gc.run(0, n2)
-            fc = {k: gc.get(k) for k in ("vx", "vz", "sxx", "szz", "sxz")}
-            coord_finite = all(bool(np.isfinite(v).all()) for v in fc.values())
-            gc.close()
-            _, pc = cpu_baseline(cfg, m, src, "coord", n2, fc)
-            pc["finite"] = coord_finite
-            parity["coord"] = pc
+            for fm, names in (("coord", ("vx", "vz", "sxx", "szz", "sxz")),
+                              ("tensor", ("vx", "vz", "sxx", "szz", "sxz", "txx", "tzz", "txz"))):
+                log(f"bench: {fm}-form bitwise check, {n2} steps")
+                gc = make(__import__("fdmod_hip").Modeling, fm, cfg, m, src)
+                gc.run(0, n2)
+                fc = {k: gc.get(k) for k in names}
+                finite = all(bool(np.isfinite(v).all()) for v in fc.values())
+                gc.close()
+                _, pc = cpu_baseline(cfg, m, src, fm, n2, fc)
+                pc["finite"] = finite
+                parity[fm] = pc
         if gradients is not None:
             log("bench: FP64 oracle gradient on a reduced replica")
             gradients["cpu_fp64_reference"] = cpu_gradient_replica()
