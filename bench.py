@@ -449,6 +449,7 @@ def main():
         fields = {k: g.get(k) for k in ("vx", "vz", "sxx", "szz", "sxz")}
         log(f"bench: CPU baseline (oracle FP32 port), {args.cpu_steps} steps")
         cpu, parity = cpu_baseline(cfg, m, src, args.form, args.cpu_steps, fields)
+        parity["finite"] = all(bool(np.isfinite(v).all()) for v in fields.values())
         parity = {args.form: parity}
         if args.form == "bond":
             # the Bond free-surface fix-up makes this marine model NaN (water at theta = 0,
@@ -460,10 +461,10 @@ def main():
                 gc = make(__import__("fdmod_hip").Modeling, fm, cfg, m, src)
                 gc.run(0, n2)
                 fc = {k: gc.get(k) for k in names}
-                finite = all(bool(np.isfinite(v).all()) for v in fc.values())
+                fc_finite = all(bool(np.isfinite(v).all()) for v in fc.values())
                 gc.close()
                 _, pc = cpu_baseline(cfg, m, src, fm, n2, fc)
-                pc["finite"] = finite
+                pc["finite"] = fc_finite
                 parity[fm] = pc
         if gradients is not None:
             log("bench: FP64 oracle gradient on a reduced replica")
