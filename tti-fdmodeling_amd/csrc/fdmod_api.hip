@@ -138,14 +138,25 @@ extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
   }
   int nused = 0;
   for (int a = 0; a < FDMOD_A_COUNT; a++) nused += form_uses(p->form, a);
+  // Every array starts kZAlign - (izStart mod kZAlign) floats into its slot, so that row
+  // izStart -- the first row of every 64-row strip -- begins a 128-byte line: a wave's
+  // 64-row access then covers two whole lines instead of straddling three that the
+  // neighbouring strips also fetch.  The slot pitch stays Nx*ldz floats; the pool keeps
+  // kZAlign spare floats at its end for the shift of the last slot.
+  c->zsh = (kZAlign - g.izStart % kZAlign) % kZAlign;
+  c->zsh16 = (2 * kZAlign - g.izStart % (2 * kZAlign)) % (2 * kZAlign);
+  if (const char *za = getenv("FDMOD_ZALIGN"))  // 0 = no shift (tuning experiments)
+    if (atoi(za) == 0) c->zsh = c->zsh16 = 0;
   size_t per = (size_t)g.Nx * g.ldz;
-  if (hipMalloc(&c->pool, per * nused * sizeof(float)) != hipSuccess) {
+  const size_t pool_n = per * nused + kZAlign;
+  if (hipMalloc(&c->pool, pool_n * sizeof(float)) != hipSuccess) {
     fdmod_destroy(c);
     return FDMOD_ENOMEM;
   }
   int k = 0;
-  for (int a = 0; a < FDMOD_A_COUNT; a++) c->d.a[a] = form_uses(p->form, a) ? c->pool + per * (k++) : nullptr;
-  if (hipMemsetAsync(c->pool, 0, per * nused * sizeof(float), c->stream) != hipSuccess ||
+  for (int a = 0; a < FDMOD_A_COUNT; a++)
+    c->d.a[a] = form_uses(p->form, a) ? c->pool + c->zsh + per * (k++) : nullptr;
+  if (hipMemsetAsync(c->pool, 0, pool_n * sizeof(float), c->stream) != hipSuccess ||
       hipMalloc(&c->flags, (size_t)g.tilesZ * g.flagsX + 1) != hipSuccess) {
     fdmod_destroy(c);
     return FDMOD_ENOMEM;
@@ -662,20 +673,34 @@ extern "C" int fdmod_time_sweeps(fdmod_ctx *c, int reps, double *ms_v, double *m
   if (!c || reps < 1) return FDMOD_EINVAL;
   if (!c->model_set) return fail(c, FDMOD_ESTATE, "fdmod_time_sweeps before fdmod_set_model");
   SweepArgs sa = sweep_args(c);
-  float ms = 0.0f;
-  HIPCALL(c, hipEventRecord(c->ev0, c->stream));
-  for (int r = 0; r < reps; r++) launch_vsweep(c->stream, c->p.form, sa);
-  HIPCALL(c, hipEventRecord(c->ev1, c->stream));
-  HIPCALL(c, hipEventSynchronize(c->ev1));
-  HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  if (ms_v) *ms_v = (double)ms / reps;
-  HIPCALL(c, hipEventRecord(c->ev0, c->stream));
-  for (int r = 0; r < reps; r++) launch_ssweep(c->stream, c->p.form, sa);
-  HIPCALL(c, hipEventRecord(c->ev1, c->stream));
-  HIPCALL(c, hipEventSynchronize(c->ev1));
-  HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  if (ms_s) *ms_s = (double)ms / reps;
-  HIPCALL(c, hipGetLastError());
+  // The sweeps alternate as in the time loop, with an event between every two launches.
+  // Timed back to back, a sweep would find part of its read-only operands still in the
+  // last-level cache, which the other sweep evicts in the loop.
+  std::vector<hipEvent_t> ev(2 * (size_t)reps + 1, nullptr);
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    e = hipEventRecord(ev[2 * r], c->stream);
+    launch_vsweep(c->stream, c->p.form, sa);
+    if (e == hipSuccess) e = hipEventRecord(ev[2 * r + 1], c->stream);
+    launch_ssweep(c->stream, c->p.form, sa);
+  }
+  if (e == hipSuccess) e = hipEventRecord(ev[2 * reps], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev[2 * reps]);
+  if (e == hipSuccess) e = hipGetLastError();
+  double tv = 0.0, ts = 0.0;
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    float a = 0.0f, b = 0.0f;
+    e = hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]);
+    tv += a;
+    ts += b;
+  }
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  if (e != hipSuccess) return fail(c, FDMOD_EDEVICE, "fdmod_time_sweeps: %s", hipGetErrorString(e));
+  if (ms_v) *ms_v = tv / reps;
+  if (ms_s) *ms_s = ts / reps;
   return FDMOD_OK;
 }
 
@@ -737,7 +762,7 @@ extern "C" int fdmod_batch_create(const fdmod_params *p, int nshots, const int *
   *out = nullptr;
   fdmod_batch *b = new fdmod_batch();
   b->p = *p;
-  b->segLS = default_segLS(p->form, kWideCols);  // the batch launches cover many shots' columns
+  b->segLS = default_segLS(p->form, 0);  // the tuner may choose another length
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
       hipMalloc(&b->d_step, sizeof(int)) != hipSuccess) {

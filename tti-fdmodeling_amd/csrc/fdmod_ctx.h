@@ -25,6 +25,8 @@ struct fdmod_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   fdm::DevArrays d;
   float *pool = nullptr;
+  int zsh = 0;    // elements each pool and gradient field is shifted by (row izStart on a 128-byte line)
+  int zsh16 = 0;  // the same for FP16 gradient storage
   uint8_t *flags = nullptr;
   bool model_set = false;
   // source

@@ -37,6 +37,18 @@ constexpr int kAdjMinTiles = 1450;            // blocks per adjoint sweep below 
 
 size_t field_floats(const fdmod_ctx *c) { return (size_t)c->g.Nx * c->g.ldz; }
 
+// Gradient field arrays (adjoint fields, FP64 sums, snapshots, history and checkpoint
+// slots) keep the row alignment of the modelling pool: a block of same-shape fields
+// holds kSpare extra elements, and its field k starts at k * field_floats + the
+// context's shift (zsh for 4- and 8-byte elements, zsh16 for FP16), so that row
+// izStart of every field begins a 128-byte line.
+constexpr size_t kSpare = 2 * kZAlign;
+template <typename T>
+T *fld(const fdmod_ctx *c, T *base, size_t k)
+{
+  return base + (sizeof(T) == 2 ? c->zsh16 : c->zsh) + k * field_floats(c);
+}
+
 const int kStateIds[kMaxState] = {FDMOD_A_VX,  FDMOD_A_VZ,  FDMOD_A_SXX, FDMOD_A_SZZ,
                                   FDMOD_A_SXZ, FDMOD_A_TXX, FDMOD_A_TZZ, FDMOD_A_TXZ};
 
@@ -60,11 +72,11 @@ int workspace(fdmod_ctx *c, bool need_G)
     c->d_mu = nullptr; c->d_G = nullptr; c->d_snap = nullptr;
     c->d_mu_n = 0;
   }
-  if (!c->d_mu) HIPCALL(c, hipMalloc(&c->d_mu, kMaxState * n * sizeof(float)));
+  if (!c->d_mu) HIPCALL(c, hipMalloc(&c->d_mu, (kMaxState * n + kSpare) * sizeof(float)));
   c->d_mu_n = n;
   if (!c->d_red) HIPCALL(c, hipMalloc(&c->d_red, (kRedBlocks + 8) * sizeof(double)));
-  if (need_G && !c->d_G) HIPCALL(c, hipMalloc(&c->d_G, kNG * n * sizeof(double)));
-  if (need_G && !c->d_snap) HIPCALL(c, hipMalloc(&c->d_snap, (size_t)kImgK * 6 * n * sizeof(float)));
+  if (need_G && !c->d_G) HIPCALL(c, hipMalloc(&c->d_G, (kNG * n + kSpare) * sizeof(double)));
+  if (need_G && !c->d_snap) HIPCALL(c, hipMalloc(&c->d_snap, ((size_t)kImgK * 6 * n + kSpare) * sizeof(float)));
   dfree(c->d_res);
   c->d_res = nullptr;
   HIPCALL(c, hipMalloc(&c->d_res, 3 * (size_t)c->nr * c->ntsample * sizeof(float)));
@@ -90,7 +102,7 @@ size_t scales_per_field(const fdmod_ctx *c) { return (size_t)c->g.Nx * (c->g.ldz
 // Capacity for nslots slots of nf fields.
 int store_reserve(fdmod_ctx *c, FieldStore &st, size_t nslots, int nf, bool f16)
 {
-  const size_t n = field_floats(c) * nslots * nf;
+  const size_t n = field_floats(c) * nslots * nf + kSpare;
   int rc = reserve(c, &st.data, st.cap_data, n * (f16 ? sizeof(_Float16) : sizeof(float)));
   if (rc || !f16) return rc;
   return reserve(c, (void **)&st.scale, st.cap_scale, scales_per_field(c) * nslots * nf * sizeof(float));
@@ -101,10 +113,11 @@ int store_io(fdmod_ctx *c, FieldStore &st, size_t slot, int nf, float *const *fi
 {
   const size_t n = field_floats(c);
   if (!f16) {
-    float *base = (float *)st.data + slot * nf * n;
-    for (int f = 0; f < nf; f++)
-      HIPCALL(c, hipMemcpyAsync(save ? base + f * n : fields[f], save ? fields[f] : base + f * n, n * sizeof(float),
+    for (int f = 0; f < nf; f++) {
+      float *slotf = fld(c, (float *)st.data, slot * nf + f);
+      HIPCALL(c, hipMemcpyAsync(save ? slotf : fields[f], save ? fields[f] : slotf, n * sizeof(float),
                                 hipMemcpyDeviceToDevice, c->stream));
+    }
     return FDMOD_OK;
   }
   Pack16 pk{};
@@ -114,7 +127,7 @@ int store_io(fdmod_ctx *c, FieldStore &st, size_t slot, int nf, float *const *fi
   const size_t ns = scales_per_field(c);
   for (int f = 0; f < nf; f++) {
     pk.src[f] = fields[f];
-    pk.dst[f] = (_Float16 *)st.data + (slot * nf + f) * n;
+    pk.dst[f] = fld(c, (_Float16 *)st.data, slot * nf + f);
     pk.inv[f] = st.scale + (slot * nf + f) * ns;
   }
   launch_pack16(c->stream, pk, save);
@@ -130,8 +143,8 @@ SweepArgs with_hist(fdmod_ctx *c, const SweepArgs &sa, size_t slot, bool f16)
   const size_t n = field_floats(c), ns = scales_per_field(c);
   r.estore = f16 ? 2 : 1;
   for (int f = 0; f < nh; f++) {
-    if (f16) r.h16[f] = (_Float16 *)c->hist.data + (slot * nh + f) * n;
-    else r.h32[f] = (float *)c->hist.data + (slot * nh + f) * n;
+    if (f16) r.h16[f] = fld(c, (_Float16 *)c->hist.data, slot * nh + f);
+    else r.h32[f] = fld(c, (float *)c->hist.data, slot * nh + f);
   }
   if (f16)
     for (int q = 0; q < 2; q++) r.hsc[q] = c->hist.scale + (slot * 2 + q) * ns;
@@ -143,13 +156,13 @@ void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
 {
   const int nh = n_hist(c->p.form);
   const size_t n = field_floats(c), ns = scales_per_field(c);
-  for (int q = 0; q < 6; q++) b.lam[j][q] = c->d_snap + ((size_t)j * 6 + q) * n;
+  for (int q = 0; q < 6; q++) b.lam[j][q] = fld(c, c->d_snap, (size_t)j * 6 + q);
   for (int f = 0; f < 8; f++) {
     b.h32[j][f] = nullptr;
     b.h16[j][f] = nullptr;
     if (f >= nh) continue;
-    if (f16) b.h16[j][f] = (const _Float16 *)c->hist.data + (slot * nh + f) * n;
-    else b.h32[j][f] = (const float *)c->hist.data + (slot * nh + f) * n;
+    if (f16) b.h16[j][f] = fld(c, (const _Float16 *)c->hist.data, slot * nh + f);
+    else b.h32[j][f] = fld(c, (const float *)c->hist.data, slot * nh + f);
   }
   for (int q = 0; q < 2; q++) b.hsc[j][q] = f16 ? c->hist.scale + (slot * 2 + q) * ns : nullptr;
 }
@@ -197,15 +210,15 @@ AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
   const size_t n = field_floats(c);
   float *const *d = c->d.a;
   a.form = c->p.form;
-  a.mvx = c->d_mu; a.mvz = c->d_mu + n; a.mxx = c->d_mu + 2 * n; a.mzz = c->d_mu + 3 * n; a.mxz = c->d_mu + 4 * n;
-  a.mtxx = c->d_mu + 5 * n; a.mtzz = c->d_mu + 6 * n; a.mtxz = c->d_mu + 7 * n;
+  a.mvx = fld(c, c->d_mu, 0); a.mvz = fld(c, c->d_mu, 1); a.mxx = fld(c, c->d_mu, 2); a.mzz = fld(c, c->d_mu, 3);
+  a.mxz = fld(c, c->d_mu, 4); a.mtxx = fld(c, c->d_mu, 5); a.mtzz = fld(c, c->d_mu, 6); a.mtxz = fld(c, c->d_mu, 7);
   a.coef1 = d[FDMOD_A_COEF1]; a.coef2 = d[FDMOD_A_COEF2]; a.invrho = d[FDMOD_A_INVRHO];
   a.c11 = d[FDMOD_A_C11]; a.c13 = d[FDMOD_A_C13]; a.c15 = d[FDMOD_A_C15];
   a.c33 = d[FDMOD_A_C33]; a.c35 = d[FDMOD_A_C35]; a.c55 = d[FDMOD_A_C55];
   a.sin2 = d[FDMOD_A_SIN2]; a.cos2 = d[FDMOD_A_COS2]; a.sin2t = d[FDMOD_A_SIN2T];
   a.sn = d[FDMOD_A_SIN]; a.cs = d[FDMOD_A_COS]; a.snv = d[FDMOD_A_SINV]; a.csv = d[FDMOD_A_COSV];
   for (int q = 0; q < 6; q++) a.snap[q] = nullptr;
-  for (int j = 0; j < kNG; j++) a.G[j] = c->d_G ? c->d_G + j * n : nullptr;
+  for (int j = 0; j < kNG; j++) a.G[j] = c->d_G ? fld(c, c->d_G, j) : nullptr;
   return a;
 }
 
@@ -281,15 +294,15 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *src
   }
   AdjArgs av = adj_args(c, true);
   if (snap_slot >= 0)
-    for (int q = 0; q < n_snap(p.form); q++) av.snap[q] = c->d_snap + ((size_t)snap_slot * 6 + q) * field_floats(c);
+    for (int q = 0; q < n_snap(p.form); q++) av.snap[q] = fld(c, c->d_snap, (size_t)snap_slot * 6 + q);
   launch_adj_vsweep(s, av);
 }
 
 int zero_adjoint(fdmod_ctx *c)
 {
   const size_t n = field_floats(c);
-  HIPCALL(c, hipMemsetAsync(c->d_mu, 0, kMaxState * n * sizeof(float), c->stream));
-  if (c->d_G) HIPCALL(c, hipMemsetAsync(c->d_G, 0, kNG * n * sizeof(double), c->stream));
+  HIPCALL(c, hipMemsetAsync(c->d_mu, 0, (kMaxState * n + kSpare) * sizeof(float), c->stream));
+  if (c->d_G) HIPCALL(c, hipMemsetAsync(c->d_G, 0, (kNG * n + kSpare) * sizeof(double), c->stream));
   return FDMOD_OK;
 }
 
@@ -396,7 +409,7 @@ int check_undamped(fdmod_ctx *c, const Recon &rg)
 int recon_negated(fdmod_ctx *c)
 {
   const size_t n = field_floats(c), ns = (size_t)c->ns * c->p.nt;
-  if (!c->d_coef2n) HIPCALL(c, hipMalloc(&c->d_coef2n, n * sizeof(float)));
+  if (!c->d_coef2n) HIPCALL(c, hipMalloc(&c->d_coef2n, (n + kSpare) * sizeof(float)));
   if (ns && c->d_srcn_n < ns) {
     dfree(c->d_srcn);
     c->d_srcn = nullptr;
@@ -404,7 +417,7 @@ int recon_negated(fdmod_ctx *c)
     HIPCALL(c, hipMalloc(&c->d_srcn, ns * sizeof(float)));
     c->d_srcn_n = ns;
   }
-  launch_negate(c->stream, c->d.a[FDMOD_A_COEF2], c->d_coef2n, n);
+  launch_negate(c->stream, c->d.a[FDMOD_A_COEF2], fld(c, c->d_coef2n, 0), n);
   if (ns) launch_negate(c->stream, c->d_src, c->d_srcn, ns);
   return FDMOD_OK;
 }
@@ -494,12 +507,13 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
     r.seg = r.nanch > 0 ? (p.nt + r.nanch) / (r.nanch + 1) : p.nt + 1;
     r.nanch = r.nanch > 0 ? (p.nt - 1) / r.seg : 0;  // states at k = seg, 2 seg, ... < nt
     const int ns = n_state(p.form), nh = n_hist(p.form);
-    const size_t st = (size_t)r.nanch * ns * field_floats(c);
+    const size_t st = (size_t)r.nanch * ns * field_floats(c) + kSpare;  // restart states, then band and frame
     const size_t fl = st + (size_t)p.nt * (ns * r.rg.nband + nh * r.rg.nframe);
     if ((rc = reserve(c, &c->ckpt.data, c->ckpt.cap_data, std::max<size_t>(fl, 1) * sizeof(float)))) return rc;
     if ((rc = store_reserve(c, c->hist, kImgK, nh, false))) return rc;
     if (interior)  // the reversed sweeps store inside R only: the rest images zero
-      HIPCALL(c, hipMemsetAsync(c->hist.data, 0, (size_t)kImgK * nh * field_floats(c) * sizeof(float), c->stream));
+      HIPCALL(c, hipMemsetAsync(c->hist.data, 0, ((size_t)kImgK * nh * field_floats(c) + kSpare) * sizeof(float),
+                                c->stream));
     r.band = (float *)c->ckpt.data + st;
     r.frame = r.band + (size_t)p.nt * ns * r.rg.nband;
   } else if (r.history) {
@@ -514,7 +528,7 @@ int grad_begin(GradRun &r, fdmod_ctx *c, const float *obs_vx, const float *obs_v
   r.sa = sweep_args(c);
   r.rsa = r.sa;
   r.rsa.g.dt = -r.sa.g.dt;
-  r.rsa.coef2 = c->d_coef2n;
+  r.rsa.coef2 = fld(c, c->d_coef2n, 0);
   r.rsa.rev = 1;
   if (r.recon && r.rg.interior) {
     r.rsa.hskip[0] = r.rg.rx0;
@@ -535,7 +549,7 @@ float *const *state_fields(fdmod_ctx *c, float **f)
 float *const *hist_fields(fdmod_ctx *c, size_t slot, float **f)
 {
   const int nh = n_hist(c->p.form);
-  for (int i = 0; i < nh; i++) f[i] = (float *)c->hist.data + (slot * nh + i) * field_floats(c);
+  for (int i = 0; i < nh; i++) f[i] = fld(c, (float *)c->hist.data, slot * nh + i);
   return f;
 }
 
@@ -751,8 +765,8 @@ int grad_finish(GradRun &r)
   const fdmod_params &p = c->p;
   if (!c->d_grad) HIPCALL(c, hipMalloc(&c->d_grad, 4 * (size_t)p.nx * p.nz * sizeof(double)));
   const int fs_row = p.freesurface && p.form == FDMOD_FORM_BOND ? c->g.e : -1;
-  if (p.form == FDMOD_FORM_COORD) launch_fold_trig(c->stream, c->g, c->d_G, field_floats(c));
-  launch_chain(c->stream, c->g, p.form, p.nx, p.nz, fs_row, c->d, c->d_G, field_floats(c),
+  if (p.form == FDMOD_FORM_COORD) launch_fold_trig(c->stream, c->g, fld(c, c->d_G, 0), field_floats(c));
+  launch_chain(c->stream, c->g, p.form, p.nx, p.nz, fs_row, c->d, fld(c, c->d_G, 0), field_floats(c),
                (double)p.scalar / 180.0 * kPI, c->d_grad);
   HIPCALL(c, hipGetLastError());
   return FDMOD_OK;

@@ -29,18 +29,14 @@ constexpr int TZ = 64;
 constexpr int kRing = 11;          // LDS ring of staged columns (10 live + 1 in flight)
 constexpr int kFlagW = 11;         // columns per fine flag tile
 constexpr int kDefaultSegLV = 44;  // measured best on a 1256 x 8002 grid
-// Stress-sweep segments.  On a wide grid short segments win, since odd segments walk
-// right to left and share their halo through L2 (1256 x 8002: Bond 8 columns 51.1
-// Gpts/s against 50.7 with 11, coordinate flat within 0.5%, tensor 4 columns 40.3
-// against 38.9); on a 1256 x 2130 crop 11 columns ran the forward 10% faster than 8.
+// Stress-sweep segments: 11 columns on every grid.  Odd segments walk right to left
+// and share their halo through L2.  Measured on 1256 x 8002 with 128-byte-aligned
+// strips (us per launch, segments of 4 / 6 / 8 / 11 / 16 columns): Bond 99.7 / 95.3 /
+// 93.5 / 92.9 / 93.6; coordinate 99.1 / 94.9 / 92.7 / 91.7; tensor 142.8 / 140.9 /
+// 137.1 / 137.6.  On a 1256 x 2130 crop 11 columns ran the forward 10% faster than 8.
 constexpr int kDefaultSegLS = 11;
-constexpr int kWideSegLS = 8, kWideSegLSTensor = 4;
-constexpr int kWideCols = 4000;  // updated columns from which a grid counts as wide
-inline int default_segLS(int form, int cols)
-{
-  if (cols < kWideCols) return kDefaultSegLS;
-  return form == FDMOD_FORM_TENSOR ? kWideSegLSTensor : kWideSegLS;
-}
+inline int default_segLS(int /*form*/, int /*cols*/) { return kDefaultSegLS; }
+constexpr int kZAlign = 32;    // floats per 128-byte line: strips start on a line
 constexpr int kMinTiles = 3000;    // blocks per sweep below which segments shorten
 // Padded grid points per shot group of a batched loop: about one full-size shot
 // (10240 columns of 1280 rows); measured on 64 crops 1126 deep, groups of 9-10k

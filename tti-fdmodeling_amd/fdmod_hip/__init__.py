@@ -188,7 +188,8 @@ class Modeling:
         return lhs.value, rhs.value
 
     def time_sweeps(self, reps=20):
-        """Mean device ms of one velocity-sweep and one stress-sweep launch."""
+        """Mean device ms of one velocity-sweep and one stress-sweep launch, the two
+        alternating as in the time loop."""
         v, s = ct.c_double(0), ct.c_double(0)
         self._check(self.L.fdmod_time_sweeps(self.c, reps, ct.byref(v), ct.byref(s)))
         return v.value, s.value
