@@ -245,11 +245,12 @@ int fdmod_adjoint_source(fdmod_ctx *ctx, const float *y_vx, const float *y_vz, c
 int fdmod_adjoint_dot(fdmod_ctx *ctx, const float *y_vx, const float *y_vz, const float *y_p, double *lhs,
                       double *rhs);
 
-/* Measurement hook (no counterpart in the reference): launches reps pairs of the
- * velocity and stress sweeps, alternating as in the time loop, on the context's stream,
- * and returns the mean duration of one launch of each from HIP events recorded on that
- * stream between the launches.  The wavefield advances; call fdmod_reset() before
- * reusing the context. */
+/* Measurement hook (no counterpart in the reference): runs min(reps, nt) loop
+ * iterations from iteration 0, as fdmod_run does, and returns the mean duration of
+ * the velocity sweep and of the stress sweep (with the loop's fused source and
+ * receiver work; any velocity-source launch counts with the stress sweep) from HIP
+ * events recorded on the context's stream between the launches.  The wavefield and
+ * the recorded traces advance; call fdmod_reset() before reusing the context. */
 int fdmod_time_sweeps(fdmod_ctx *ctx, int reps, double *ms_v, double *ms_s);
 
 #ifdef __cplusplus

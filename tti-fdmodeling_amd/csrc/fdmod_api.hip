@@ -418,10 +418,14 @@ extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, c
     for (int i = 0; i < nr; i++) start[gx[i] + 1]++;
     for (int x = 0; x < Nx; x++) start[x + 1] += start[x];
     std::vector<int> fill(start.begin(), start.end() - 1);
+    c->rc_zlo = INT32_MAX;
+    c->rc_zhi = -1;
     for (int i = 0; i < nr; i++) {
       const int k = fill[gx[i]]++;
       rz[k] = gz[i];
       rid[k] = i;
+      c->rc_zlo = std::min(c->rc_zlo, gz[i]);
+      c->rc_zhi = std::max(c->rc_zhi, gz[i]);
     }
     HIPCALL(c, hipMalloc(&c->d_rc_start, (Nx + 1) * sizeof(int)));
     HIPCALL(c, hipMalloc(&c->d_rc_z, nr * sizeof(int)));
@@ -519,9 +523,10 @@ static int snapshot(fdmod_ctx *c, int it)
 
 // One iteration of the reference loop (BT:651-954): V sweep, velocity sources,
 // S sweep (+ free surface), stress source, snapshot, receivers, energy.
-int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
+int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record, const hipEvent_t *mark)
 {
   const fdmod_params &p = c->p;
+  if (mark) HIPCALL(c, hipEventRecord(mark[0], c->stream));
   launch_vsweep(c->stream, p.form, sa);
   SourceArgs src;
   if (c->ns > 0) {
@@ -536,6 +541,7 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
     src.fb = (c->stype == 5 || c->stype == 6) ? nullptr : sa.vz;
     launch_source(c->stream, src);
   }
+  if (mark) HIPCALL(c, hipEventRecord(mark[1], c->stream));
   const bool snap_now = record && c->wsave > 0 && it % c->snap_sample == 0;
   const bool rec_now = record && c->seistype > 0 && c->nr > 0 && it % c->mt == 0;
   const bool fsrc = c->fuse_src && c->ns > 0;
@@ -555,6 +561,8 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
       sf.rc_start = c->d_rc_start;
       sf.rc_z = c->d_rc_z;
       sf.rc_id = c->d_rc_id;
+      sf.rc_zlo = c->rc_zlo;
+      sf.rc_zhi = c->rc_zhi;
       sf.rec_p = c->d_rp;
       sf.rec_off = (size_t)(it / c->mt) * c->nr;
     }
@@ -562,6 +570,7 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record)
   } else {
     launch_ssweep(c->stream, p.form, sa);
   }
+  if (mark) HIPCALL(c, hipEventRecord(mark[2], c->stream));
   RecvArgs r{};
   if (rec_now && !frec) {
     r.nr = c->nr; r.ldz = c->g.ldz; r.gx = c->d_gx; r.gz = c->d_gz;
@@ -672,35 +681,35 @@ extern "C" int fdmod_time_sweeps(fdmod_ctx *c, int reps, double *ms_v, double *m
 {
   if (!c || reps < 1) return FDMOD_EINVAL;
   if (!c->model_set) return fail(c, FDMOD_ESTATE, "fdmod_time_sweeps before fdmod_set_model");
+  const fdmod_params &p = c->p;
   SweepArgs sa = sweep_args(c);
-  // The sweeps alternate as in the time loop, with an event between every two launches.
-  // Timed back to back, a sweep would find part of its read-only operands still in the
-  // last-level cache, which the other sweep evicts in the loop.
-  std::vector<hipEvent_t> ev(2 * (size_t)reps + 1, nullptr);
+  // reps loop iterations from iteration 0, each sweep bracketed by events: the sweeps
+  // alternate and carry the loop's fused source and receiver work, as in fdmod_run.
+  // (Timed back to back, a sweep would find part of its read-only operands still in the
+  // last-level cache, which the other sweep evicts in the loop.)
+  const int n = std::min(reps, p.nt);
+  std::vector<hipEvent_t> ev(3 * (size_t)n, nullptr);
   hipError_t e = hipSuccess;
+  int rc = FDMOD_OK;
   for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
-  for (int r = 0; r < reps && e == hipSuccess; r++) {
-    e = hipEventRecord(ev[2 * r], c->stream);
-    launch_vsweep(c->stream, c->p.form, sa);
-    if (e == hipSuccess) e = hipEventRecord(ev[2 * r + 1], c->stream);
-    launch_ssweep(c->stream, c->p.form, sa);
-  }
-  if (e == hipSuccess) e = hipEventRecord(ev[2 * reps], c->stream);
-  if (e == hipSuccess) e = hipEventSynchronize(ev[2 * reps]);
-  if (e == hipSuccess) e = hipGetLastError();
+  for (int k = 0; k < n && e == hipSuccess && rc == FDMOD_OK; k++)
+    rc = step(c, p.mode == 1 ? k : p.nt - 1 - k, sa, true, &ev[3 * (size_t)k]);
+  if (e == hipSuccess && rc == FDMOD_OK) e = hipEventSynchronize(ev[3 * (size_t)n - 1]);
+  if (e == hipSuccess && rc == FDMOD_OK) e = hipGetLastError();
   double tv = 0.0, ts = 0.0;
-  for (int r = 0; r < reps && e == hipSuccess; r++) {
+  for (int k = 0; k < n && e == hipSuccess && rc == FDMOD_OK; k++) {
     float a = 0.0f, b = 0.0f;
-    e = hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]);
+    e = hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]);
     tv += a;
     ts += b;
   }
   for (hipEvent_t x : ev)
     if (x) (void)hipEventDestroy(x);
+  if (rc != FDMOD_OK) return rc;
   if (e != hipSuccess) return fail(c, FDMOD_EDEVICE, "fdmod_time_sweeps: %s", hipGetErrorString(e));
-  if (ms_v) *ms_v = tv / reps;
-  if (ms_s) *ms_s = ts / reps;
+  if (ms_v) *ms_v = tv / n;
+  if (ms_s) *ms_s = ts / n;
   return FDMOD_OK;
 }
 

@@ -39,6 +39,7 @@ struct fdmod_ctx {
   // receivers
   int seistype = 0, mt = 1, nr = 0, ntsample = 0;
   int *d_rc_start = nullptr, *d_rc_z = nullptr, *d_rc_id = nullptr;  // receivers by column (CSR)
+  int rc_zlo = 0, rc_zhi = -1;  // padded rows spanned by the receivers
   int *d_gx = nullptr, *d_gz = nullptr;
   float *d_rvx = nullptr, *d_rvz = nullptr, *d_rp = nullptr;
   // snapshots
@@ -128,4 +129,6 @@ int bfail(fdmod_batch *b, int code, const char *fmt, ...);
 
 fdm::SweepArgs sweep_args(fdmod_ctx *c);
 // One iteration of the loop; record = false skips snapshots, receivers and energy.
-int step(fdmod_ctx *c, int it, const fdm::SweepArgs &sa, bool record = true);
+// One loop iteration.  mark, if given, holds three events recorded before the velocity
+// sweep, between the sweeps (after any velocity source) and after the stress sweep.
+int step(fdmod_ctx *c, int it, const fdm::SweepArgs &sa, bool record = true, const hipEvent_t *mark = nullptr);

@@ -92,12 +92,14 @@ struct SweepArgs {
   // Single-shot loop, Bond and coordinate forms: the stress sweep also adds the stress
   // source (fuse bit 1: stype 1, one source whose 11 x 11 footprint the sweep writes;
   // ftap = its tap table, ordered ox-major) and records the pressure receivers (bit 2:
-  // rc_* = receivers by padded column, CSR; rec_p[rec_off + id]).
+  // rc_* = receivers by padded column, CSR; rec_p[rec_off + id]; rows rc_zlo..rc_zhi
+  // hold every receiver, so other strips skip the lookup).
   int fuse;
   int fsx, fsz, fit;
   const struct SrcTap *ftap;
   const float *fsrc;
   const int *rc_start, *rc_z, *rc_id;
+  int rc_zlo, rc_zhi;
   float *rec_p;
   size_t rec_off;
 };

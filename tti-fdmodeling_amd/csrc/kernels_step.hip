@@ -53,25 +53,17 @@ struct VOps {
   float vx, vz, ir, k1, k2, cv, sv;
 };
 
-#ifndef FDMOD_VOPS_NT
-#define FDMOD_VOPS_NT 1
-#endif
-// Cache policy of the velocity sweep's per-point operands (1 = non-temporal).  Measured on
-// the 1256 x 8002 grid with 128-byte-aligned strips: coordinate form 72.0 -> 65.8 us per
-// launch (it reads two more operands), Bond and tensor unchanged.
-__device__ __forceinline__ float ldv(const float *p) { return FDMOD_VOPS_NT ? ldnt(p) : ldst(p); }
-
 template <bool COORD>
 __device__ __forceinline__ void load_vops(VOps &p, const SweepArgs &a, size_t o, bool und, float dt)
 {
-  p.vx = ldv(a.vx + o);
-  p.vz = ldv(a.vz + o);
-  p.ir = ldv(a.invrho + o);
-  p.k1 = und ? 1.0f : ldv(a.coef1 + o);
-  p.k2 = und ? dt : ldv(a.coef2 + o);
+  p.vx = ldnt(a.vx + o);
+  p.vz = ldnt(a.vz + o);
+  p.ir = ldnt(a.invrho + o);
+  p.k1 = und ? 1.0f : ldnt(a.coef1 + o);
+  p.k2 = und ? dt : ldnt(a.coef2 + o);
   if (COORD) {
-    p.cv = ldv(a.csv + o);
-    p.sv = ldv(a.snv + o);
+    p.cv = ldnt(a.csv + o);
+    p.sv = ldnt(a.snv + o);
   } else {
     p.cv = p.sv = 0.0f;
   }
@@ -352,7 +344,8 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
         st_out(a.szz + o, nzz);
         st_out(a.sxz + o, nxz);
       }
-      if (a.fuse & 2)  // pressure receivers of this column (isoelastic.c:103-144)
+      // pressure receivers of this column (isoelastic.c:103-144), in strips that hold some
+      if ((a.fuse & 2) && z0 <= a.rc_zhi && z0 + TZ > a.rc_zlo)
         for (int j = a.rc_start[c]; j < a.rc_start[c + 1]; j++)
           if (zok && a.rc_z[j] == iz) a.rec_p[a.rec_off + a.rc_id[j]] = 0.5f * (nxx + nzz);
       if (ESTORE) {

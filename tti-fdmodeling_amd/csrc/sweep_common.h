@@ -11,11 +11,12 @@ namespace fdm {
 constexpr int LZ = TZ + 2 * kN - 1;  // staged rows per column: 64 + 9 halo
 constexpr int kHalo = LZ - TZ;       // rows loaded by the first kHalo lanes a second time
 
-// Cache policy (measured on MI355X, 1256 x 8002 grid): default caching for the
-// staged stencil columns, the velocity sweep's operands and all outputs; the stress
-// sweep's read-once operands (old stress, stiffness, coefs) load non-temporally.
-// Making every stream non-temporal was slowest (V 105 us, S 170 us); this mix gave
-// V 65 us and S 116 us.
+// Cache policy (measured on MI355X in the time loop, 1256 x 8002 grid, strips on
+// 128-byte lines): default caching for the staged stencil columns and all outputs;
+// both sweeps' per-point operands load non-temporally.  Alternatives measured:
+// caching the stress sweep's operands, +9 us per stress launch (95 -> 104 us); caching
+// the velocity sweep's operands, coordinate form +6 us per velocity launch; non-temporal
+// stores of either sweep, 1-4% slower steps; every stream non-temporal, slowest.
 __device__ __forceinline__ float ldnt(const float *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ float ldst(const float *p) { return *p; }
 __device__ __forceinline__ void st_out(float *p, float v) { *p = v; }
