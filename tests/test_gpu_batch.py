@@ -28,22 +28,22 @@ def _shots():
     return shots
 
 
-def _single(cls, form, sh, **kw):
+def _single(cls, form, sh, seis=3, **kw):
     o = cls(form, NZ, sh["w"], NT, 10.0, 10.0, DT, ealThick=8, freesurface=1, fpeak=15.0, **kw)
     m = sh["m"]
     o.set_model(m["vp"], m["vs"], m["rho"], m["eps"], m["delta"], m["theta"])
     o.set_source(sh["stype"], sh["sx"], sh["sz"], sh["src"])
-    o.set_receivers(3, 2, sh["rx"], sh["rz"])
+    o.set_receivers(seis, 2, sh["rx"], sh["rz"])
     return o
 
 
-def _batch(form, shots):
+def _batch(form, shots, seis=3):
     b = Batch(form, NZ, [sh["w"] for sh in shots], NT, 10.0, 10.0, DT, ealThick=8, freesurface=1, fpeak=15.0)
     for s, sh in enumerate(shots):
         m = sh["m"]
         b.set_model(s, m["vp"], m["vs"], m["rho"], m["eps"], m["delta"], m["theta"])
         b.set_source(s, sh["stype"], sh["sx"], sh["sz"], sh["src"])
-        b.set_receivers(s, 3, 2, sh["rx"], sh["rz"])
+        b.set_receivers(s, seis, 2, sh["rx"], sh["rz"])
     return b
 
 
@@ -70,6 +70,25 @@ def test_batch_equals_single_shots(form, group_pts, monkeypatch):
             tb = b.traces(s, comp)
             assert np.array_equal(tb, g.traces(comp)), (s, comp)
             assert np.array_equal(tb, o.traces(comp).astype(np.float32)), (s, comp)
+
+
+# pressure-only receivers: the Bond and coordinate batched stress sweeps record them
+# (every second iteration, mt = 2) together with the fused stype-1 sources
+@pytest.mark.parametrize("form", ["bond", "tensor", "coord"])
+def test_batch_pressure_receivers_equal_single_shots(form):
+    shots = _shots()
+    print("batch p receivers", form, flush=True)
+    b = _batch(form, shots, seis=2)
+    b.run(0, 70)
+    b.run(70, 80)
+    for s, sh in enumerate(shots):
+        g = _single(Modeling, form, sh, seis=2)
+        g.run()
+        o = _single(Oracle, form, sh, seis=2)
+        o.run()
+        tb = b.traces(s, 4)
+        assert np.array_equal(tb, g.traces(4)), s
+        assert np.array_equal(tb, o.traces(4).astype(np.float32)), s
 
 
 def test_autotuner_and_tile_shapes_keep_results():

@@ -863,7 +863,8 @@ static int batch_step(fdmod_batch *b, int g, int off)
   launch_vsweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startV + s0 + g, n, b->totV[g]);
   if (b->srcBlocksV)
     launch_source_batch(b->stream, b->d_srcV + s0, n, b->srcBlocksV, b->d_step, off, b->p.nt, b->p.mode);
-  launch_ssweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startS + s0 + g, n, b->totS[g]);
+  launch_ssweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startS + s0 + g, n, b->totS[g], b->d_step, off,
+                      b->p.nt, b->p.mode);
   if (b->srcBlocksS)
     launch_source_batch(b->stream, b->d_srcS + s0, n, b->srcBlocksS, b->d_step, off, b->p.nt, b->p.mode);
   if (b->maxnr) launch_receivers_batch(b->stream, b->d_rec + s0, n, b->maxnr, b->d_step, off, b->p.nt, b->p.mode);
@@ -921,11 +922,34 @@ static int batch_prepare(fdmod_batch *b, bool with_graph)
     b->totS[g] = startS[k + 1];
     sv[s] = shot_source(c, sw[s], false);
     ss[s] = shot_source(c, sw[s], true);
+    // as in the single-shot loop, the stress sweep adds a single covered stype-1 source
+    // and records seistype-2 receivers; the sweep reads the iteration on the device
+    const bool fsrc = c->fuse_src && c->ns > 0;
+    const bool frec = c->seistype == 2 && c->nr > 0 && c->d_rc_start && b->p.form != FDMOD_FORM_TENSOR;
+    if (fsrc) {
+      sw[s].fuse |= 1;
+      sw[s].fsx = c->src_cell_x;
+      sw[s].fsz = c->src_cell_z;
+      sw[s].ftap = c->d_taps;
+      sw[s].fsrc = c->d_src;
+      ss[s].ns = 0;
+    }
+    if (frec) {
+      sw[s].fuse |= 2;
+      sw[s].rc_start = c->d_rc_start;
+      sw[s].rc_z = c->d_rc_z;
+      sw[s].rc_id = c->d_rc_id;
+      sw[s].rc_zlo = c->rc_zlo;
+      sw[s].rc_zhi = c->rc_zhi;
+      sw[s].rec_p = c->d_rp;
+      sw[s].rc_mt = c->mt;
+      sw[s].rc_nr = c->nr;
+    }
     if (sv[s].ns) b->srcBlocksV = std::max(b->srcBlocksV, sv[s].sequential ? 1 : sv[s].ns);
     if (ss[s].ns) b->srcBlocksS = std::max(b->srcBlocksS, ss[s].sequential ? 1 : ss[s].ns);
     RecvArgs &r = rv[s];
     memset(&r, 0, sizeof r);
-    if (c->seistype > 0 && c->nr > 0) {
+    if (c->seistype > 0 && c->nr > 0 && !frec) {
       r.nr = c->nr; r.ldz = c->g.ldz; r.gx = c->d_gx; r.gz = c->d_gz;
       r.vx = sw[s].vx; r.vz = sw[s].vz; r.sxx = sw[s].sxx; r.szz = sw[s].szz;
       r.out_vx = c->d_rvx; r.out_vz = c->d_rvz; r.out_p = c->d_rp;
@@ -1018,34 +1042,34 @@ extern "C" int fdmod_batch_get_traces(fdmod_batch *b, int shot, int comp, float 
   return FDMOD_OK;
 }
 
-// Autotuner: times candidate segment lengths of both sweeps on this batch (each sweep
-// launched `reps` times per candidate, HIP events on the batch stream) and keeps the
-// fastest of each.  The wavefields advance during tuning and are reset afterwards.
+// Autotuner: times candidate segment lengths of both sweeps on this batch and keeps the
+// fastest of each.  Each candidate runs min(reps, nt) whole loop iterations from
+// iteration 0, as the run launches them (HIP events on the batch stream): a sweep timed
+// back to back would find operands in the last-level cache that the loop evicts.  The
+// wavefields advance during tuning and are reset afterwards.
 extern "C" int fdmod_batch_tune(fdmod_batch *b, int reps, int *segLV, int *segLS)
 {
   if (!b || reps < 1) return FDMOD_EINVAL;
   static const int candV[] = {22, 33, 44, 55, 66, 88};
   static const int candS[] = {4, 5, 8, 11, 22};
+  const int n = std::min(reps, b->p.nt);
+  const int zero = 0;
   auto time_one = [&](bool vsweep, int L, double &ms_out) -> int {
     if (vsweep) b->segLV = L;
     else b->segLS = L;
     int rc = batch_prepare(b, false);
     if (rc) return rc;
-    auto sweep = [&]() {  // one sweep of every group, as the loop launches it
-      for (size_t g = 0; g + 1 < b->grp.size(); g++) {
-        const int s0 = b->grp[g], ns = b->grp[g + 1] - s0;
-        if (vsweep) launch_vsweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startV + s0 + g, ns, b->totV[g]);
-        else launch_ssweep_batch(b->stream, b->p.form, b->d_sweep + s0, b->d_startS + s0 + g, ns, b->totS[g]);
-      }
-    };
-    for (int w = 0; w < 2; w++) sweep();  // warm-up launches
+    BHIP(b, hipMemcpyAsync(b->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    batch_steps(b, std::min(2, n));  // warm-up iterations
+    BHIP(b, hipMemcpyAsync(b->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipEventRecord(b->ev0, b->stream));
-    for (int r = 0; r < reps; r++) sweep();
+    batch_steps(b, n);
     BHIP(b, hipEventRecord(b->ev1, b->stream));
     BHIP(b, hipEventSynchronize(b->ev1));
+    BHIP(b, hipGetLastError());
     float ms = 0.0f;
     BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
-    ms_out = ms / reps;
+    ms_out = ms / n;
     return FDMOD_OK;
   };
   double best = 1e30;

@@ -100,6 +100,7 @@ struct SweepArgs {
   const float *fsrc;
   const int *rc_start, *rc_z, *rc_id;
   int rc_zlo, rc_zhi;
+  int rc_mt, rc_nr;  // batched loop: the sweep takes fit and rec_off from the step counter
   float *rec_p;
   size_t rec_off;
 };
@@ -197,7 +198,8 @@ void launch_ssweep(hipStream_t s, int form, const SweepArgs &a);
 int sweep_tiles(const Grid &g, int segL);
 // batched launches over many shots: device arrays of per-shot args, tile prefix sums
 void launch_vsweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total);
-void launch_ssweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total);
+void launch_ssweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total,
+                         const int *step, int off, int nt, int mode);
 void launch_source_batch(hipStream_t s, const SourceArgs *shots, int nshots, int maxblocks, const int *step, int off,
                          int nt, int mode);
 void launch_receivers_batch(hipStream_t s, const RecvArgs *shots, int nshots, int maxnr, const int *step, int off,

@@ -406,6 +406,12 @@ __global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
   else ssweep_tile<FORM, PD, ESTORE, false>(a, btz, btx, ring);
 }
 
+__device__ __forceinline__ int step_it(const int *step, int off, int nt, int mode)
+{
+  const int k = *step + off;
+  return mode == 1 ? k : nt - 1 - k;
+}
+
 // Batched grids: one launch covers the tiles of every shot.  start[s] is the first
 // global tile of shot s (prefix sum over shots, start[nshots] = total); the global
 // tile index is XCD-aware as in the single grid, then located by binary search.
@@ -438,14 +444,25 @@ __global__ __launch_bounds__(64) void k_vsweep_batch(const SweepArgs *__restrict
   vsweep_tile<COORD, PD>(a, lt - btx * a.g.tilesZ, btx);
 }
 
+// Shots with a fused stress source or pressure receivers (as in the single-shot loop)
+// take the iteration from the device step counter: the source sample and, when
+// it % mt == 0, the receivers' output row.
 template <int FORM, int PD>
 __global__ __launch_bounds__(64) void k_ssweep_batch(const SweepArgs *__restrict__ shots, const int *__restrict__ start,
-                                                     int nshots)
+                                                     int nshots, const int *step, int off, int nt, int mode)
 {
   __shared__ float ring[2][kRing][LZ];
   int s, lt;
   if (!batch_tile(start, nshots, s, lt)) return;
-  const SweepArgs a = shots[s];
+  SweepArgs a = shots[s];
+  if (a.fuse) {
+    const int it = step_it(step, off, nt, mode);
+    a.fit = it;
+    if (a.fuse & 2) {  // rc_mt >= 1 whenever bit 2 is set
+      if (it % a.rc_mt == 0) a.rec_off = (size_t)(it / a.rc_mt) * a.rc_nr;
+      else a.fuse &= ~2;
+    }
+  }
   const int btx = lt / a.g.tilesZ;
   if (btx & 1) ssweep_tile<FORM, PD, 0, true>(a, lt - btx * a.g.tilesZ, btx, ring);
   else ssweep_tile<FORM, PD, 0, false>(a, lt - btx * a.g.tilesZ, btx, ring);
@@ -493,14 +510,19 @@ void launch_vsweep_batch(hipStream_t s, int form, const SweepArgs *shots, const 
   else hipLaunchKernelGGL((k_vsweep_batch<false, kPipeV>), grid, dim3(64), 0, s, shots, start, nshots);
 }
 
-void launch_ssweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total)
+void launch_ssweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total,
+                         const int *step, int off, int nt, int mode)
 {
   dim3 grid((total + kXCD - 1) / kXCD * kXCD);
   if (form == FDMOD_FORM_COORD)
-    hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_COORD, pipe_s<FDMOD_FORM_COORD>()>), grid, dim3(64), 0, s, shots, start, nshots);
+    hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_COORD, pipe_s<FDMOD_FORM_COORD>()>), grid, dim3(64), 0, s, shots, start,
+                       nshots, step, off, nt, mode);
   else if (form == FDMOD_FORM_TENSOR)
-    hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_TENSOR, pipe_s<FDMOD_FORM_TENSOR>()>), grid, dim3(64), 0, s, shots, start, nshots);
-  else hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_BOND, pipe_s<FDMOD_FORM_BOND>()>), grid, dim3(64), 0, s, shots, start, nshots);
+    hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_TENSOR, pipe_s<FDMOD_FORM_TENSOR>()>), grid, dim3(64), 0, s, shots,
+                       start, nshots, step, off, nt, mode);
+  else
+    hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_BOND, pipe_s<FDMOD_FORM_BOND>()>), grid, dim3(64), 0, s, shots, start,
+                       nshots, step, off, nt, mode);
 }
 
 // Source injection (BT:706-834 on velocity, BT:893-914 on stress).  Each pass is a
@@ -545,11 +567,6 @@ void launch_source(hipStream_t s, const SourceArgs &a)
 }
 
 // Time index of loop iteration k = *step + off (BT:651): it = k or nt-1-k.
-__device__ __forceinline__ int step_it(const int *step, int off, int nt, int mode)
-{
-  const int k = *step + off;
-  return mode == 1 ? k : nt - 1 - k;
-}
 
 // Batched sources: blockIdx.y = shot; shots without a source in this phase have ns = 0.
 __global__ void k_source_batch(const SourceArgs *__restrict__ shots, const int *step, int off, int nt, int mode)
