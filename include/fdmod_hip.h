@@ -249,8 +249,9 @@ int fdmod_adjoint_dot(fdmod_ctx *ctx, const float *y_vx, const float *y_vz, cons
  * iterations from iteration 0, as fdmod_run does, and returns the mean duration of
  * the velocity sweep and of the stress sweep (with the loop's fused source and
  * receiver work; any velocity-source launch counts with the stress sweep) from HIP
- * events recorded on the context's stream between the launches.  The wavefield and
- * the recorded traces advance; call fdmod_reset() before reusing the context. */
+ * events recorded on the context's stream between the launches.  No snapshots are
+ * written.  The wavefield and the recorded traces advance; call fdmod_reset() before
+ * reusing the context. */
 int fdmod_time_sweeps(fdmod_ctx *ctx, int reps, double *ms_v, double *ms_s);
 
 #ifdef __cplusplus

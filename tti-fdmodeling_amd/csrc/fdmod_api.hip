@@ -692,8 +692,11 @@ extern "C" int fdmod_time_sweeps(fdmod_ctx *c, int reps, double *ms_v, double *m
   hipError_t e = hipSuccess;
   int rc = FDMOD_OK;
   for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
+  const int wsave = c->wsave;  // no snapshot files while timing
+  c->wsave = 0;
   for (int k = 0; k < n && e == hipSuccess && rc == FDMOD_OK; k++)
     rc = step(c, p.mode == 1 ? k : p.nt - 1 - k, sa, true, &ev[3 * (size_t)k]);
+  c->wsave = wsave;
   if (e == hipSuccess && rc == FDMOD_OK) e = hipEventSynchronize(ev[3 * (size_t)n - 1]);
   if (e == hipSuccess && rc == FDMOD_OK) e = hipGetLastError();
   double tv = 0.0, ts = 0.0;
