@@ -75,7 +75,7 @@ static void release(fdmod_ctx *c)
   dfree(c->d_gx); dfree(c->d_gz); dfree(c->d_rvx); dfree(c->d_rvz); dfree(c->d_rp);
   dfree(c->d_rc_start); dfree(c->d_rc_z); dfree(c->d_rc_id);
   dfree(c->d_p); dfree(c->d_E); dfree(c->d_Ek); dfree(c->d_Ep); dfree(c->d_partial);
-  dfree(c->d_mu); dfree(c->d_G); dfree(c->d_snap); dfree(c->d_res); dfree(c->d_red);
+  dfree(c->d_mu); dfree(c->d_mus); dfree(c->d_G); dfree(c->d_snap); dfree(c->d_res); dfree(c->d_red);
   dfree(c->ckpt.data); dfree(c->ckpt.scale); dfree(c->hist.data); dfree(c->hist.scale);
   dfree(c->d_srcadj); dfree(c->d_grad); dfree(c->d_coef2n); dfree(c->d_srcn);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -756,7 +756,7 @@ extern "C" void fdmod_batch_destroy(fdmod_batch *b)
   batch_unprepare(b);
   dfree(b->d_step);
   for (GradMem &m : b->gmem) {
-    dfree(m.mu); dfree(m.G); dfree(m.snap);
+    dfree(m.mu); dfree(m.mus); dfree(m.G); dfree(m.snap);
     dfree(m.ckpt.data); dfree(m.ckpt.scale); dfree(m.hist.data); dfree(m.hist.scale);
   }
   for (fdmod_ctx *c : b->shots) fdmod_destroy(c);

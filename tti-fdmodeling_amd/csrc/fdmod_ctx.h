@@ -54,6 +54,8 @@ struct fdmod_ctx {
   // adjoint / gradient workspace (allocated on first use, kept until destroy)
   float *d_mu = nullptr;       // five adjoint fields, each Nx*ldz
   double *d_G = nullptr;       // six imaging sums, each Nx*ldz
+  float *d_mus = nullptr;      // adjoint stress ring: kImgK + 1 slots x (mxx, mzz, mxz), Nx*ldz each
+  int mus_cur = 0, mus_last = 0;  // slot the next adjoint step starts from; slot the last one imaged
   float *d_snap = nullptr;     // mu_sigma snapshots of one imaging block (kImgK x 3 fields)
   size_t d_mu_n = 0;           // field size (floats) d_mu, d_G and d_snap were allocated for
   FieldStore ckpt;             // checkpointed states (5 fields per slot)
@@ -84,6 +86,7 @@ void dfree(void *p);  // hipFree that ignores null
 // (re-allocating tens of GB for every group of shots costs more than the gradient).
 struct GradMem {
   float *mu = nullptr;
+  float *mus = nullptr;
   double *G = nullptr;
   float *snap = nullptr;
   size_t n = 0;  // field size the three buffers above were allocated for

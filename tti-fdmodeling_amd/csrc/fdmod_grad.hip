@@ -49,6 +49,13 @@ T *fld(const fdmod_ctx *c, T *base, size_t k)
   return base + (sizeof(T) == 2 ? c->zsh16 : c->zsh) + k * field_floats(c);
 }
 
+// The adjoint stress rotates through kMusSlots slots: the V-adjoint of each step reads
+// mu_sigma from the current slot and writes the updated one into the next, so the slot
+// it read is the imaging snapshot of that step, kept until the block's imaging has run
+// (at most kImgK steps later; the slot is rewritten kImgK + 1 steps later).
+constexpr int kMusSlots = kImgK + 1;
+float *mus_field(const fdmod_ctx *c, int slot, int q) { return fld(c, c->d_mus, (size_t)slot * 3 + q); }
+
 const int kStateIds[kMaxState] = {FDMOD_A_VX,  FDMOD_A_VZ,  FDMOD_A_SXX, FDMOD_A_SZZ,
                                   FDMOD_A_SXZ, FDMOD_A_TXX, FDMOD_A_TZZ, FDMOD_A_TXZ};
 
@@ -68,11 +75,12 @@ int workspace(fdmod_ctx *c, bool need_G)
 {
   const size_t n = field_floats(c);
   if (c->d_mu_n < n) {  // (re)size the per-field workspace, which a batch may lend
-    dfree(c->d_mu); dfree(c->d_G); dfree(c->d_snap);
-    c->d_mu = nullptr; c->d_G = nullptr; c->d_snap = nullptr;
+    dfree(c->d_mu); dfree(c->d_mus); dfree(c->d_G); dfree(c->d_snap);
+    c->d_mu = nullptr; c->d_mus = nullptr; c->d_G = nullptr; c->d_snap = nullptr;
     c->d_mu_n = 0;
   }
   if (!c->d_mu) HIPCALL(c, hipMalloc(&c->d_mu, (kMaxState * n + kSpare) * sizeof(float)));
+  if (!c->d_mus) HIPCALL(c, hipMalloc(&c->d_mus, ((size_t)kMusSlots * 3 * n + kSpare) * sizeof(float)));
   c->d_mu_n = n;
   if (!c->d_red) HIPCALL(c, hipMalloc(&c->d_red, (kRedBlocks + 8) * sizeof(double)));
   if (need_G && !c->d_G) HIPCALL(c, hipMalloc(&c->d_G, (kNG * n + kSpare) * sizeof(double)));
@@ -156,7 +164,7 @@ void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
 {
   const int nh = n_hist(c->p.form);
   const size_t n = field_floats(c), ns = scales_per_field(c);
-  for (int q = 0; q < 6; q++) b.lam[j][q] = fld(c, c->d_snap, (size_t)j * 6 + q);
+  for (int q = 0; q < 6; q++) b.lam[j][q] = q < 3 ? mus_field(c, c->mus_last, q) : fld(c, c->d_snap, (size_t)j * 6 + q);
   for (int f = 0; f < 8; f++) {
     b.h32[j][f] = nullptr;
     b.h16[j][f] = nullptr;
@@ -210,8 +218,10 @@ AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
   const size_t n = field_floats(c);
   float *const *d = c->d.a;
   a.form = c->p.form;
-  a.mvx = fld(c, c->d_mu, 0); a.mvz = fld(c, c->d_mu, 1); a.mxx = fld(c, c->d_mu, 2); a.mzz = fld(c, c->d_mu, 3);
-  a.mxz = fld(c, c->d_mu, 4); a.mtxx = fld(c, c->d_mu, 5); a.mtzz = fld(c, c->d_mu, 6); a.mtxz = fld(c, c->d_mu, 7);
+  a.mvx = fld(c, c->d_mu, 0); a.mvz = fld(c, c->d_mu, 1);
+  a.mxx = mus_field(c, c->mus_cur, 0); a.mzz = mus_field(c, c->mus_cur, 1); a.mxz = mus_field(c, c->mus_cur, 2);
+  a.oxx = a.mxx; a.ozz = a.mzz; a.oxz = a.mxz;
+  a.mtxx = fld(c, c->d_mu, 5); a.mtzz = fld(c, c->d_mu, 6); a.mtxz = fld(c, c->d_mu, 7);
   a.coef1 = d[FDMOD_A_COEF1]; a.coef2 = d[FDMOD_A_COEF2]; a.invrho = d[FDMOD_A_INVRHO];
   a.c11 = d[FDMOD_A_C11]; a.c13 = d[FDMOD_A_C13]; a.c15 = d[FDMOD_A_C15];
   a.c33 = d[FDMOD_A_C33]; a.c35 = d[FDMOD_A_C35]; a.c55 = d[FDMOD_A_C55];
@@ -267,8 +277,10 @@ int upload_data(fdmod_ctx *c, const float *hvx, const float *hvz, const float *h
   return FDMOD_OK;
 }
 
-// One adjoint step: the transpose of loop iteration k.
-// snap_slot >= 0: the V-adjoint copies the imaged mu_sigma into that snapshot slot.
+// One adjoint step: the transpose of loop iteration k.  The V-adjoint moves mu_sigma to
+// the next ring slot; the slot it read (mus_last) holds this step's imaged mu_sigma.
+// snap_slot >= 0: it also stores the other imaged fields (tensor and coordinate forms)
+// into that snapshot slot.
 void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *srcadj)
 {
   const fdmod_params &p = c->p;
@@ -293,15 +305,23 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *src
     launch_adj_source(s, source_args(c, it), ma, mb, nullptr, nullptr, srcadj);
   }
   AdjArgs av = adj_args(c, true);
+  const int nxt = (c->mus_cur + 1) % kMusSlots;
+  av.oxx = mus_field(c, nxt, 0);
+  av.ozz = mus_field(c, nxt, 1);
+  av.oxz = mus_field(c, nxt, 2);
   if (snap_slot >= 0)
-    for (int q = 0; q < n_snap(p.form); q++) av.snap[q] = fld(c, c->d_snap, (size_t)snap_slot * 6 + q);
+    for (int q = 3; q < n_snap(p.form); q++) av.snap[q] = fld(c, c->d_snap, (size_t)snap_slot * 6 + q);
   launch_adj_vsweep(s, av);
+  c->mus_last = c->mus_cur;
+  c->mus_cur = nxt;
 }
 
 int zero_adjoint(fdmod_ctx *c)
 {
   const size_t n = field_floats(c);
   HIPCALL(c, hipMemsetAsync(c->d_mu, 0, (kMaxState * n + kSpare) * sizeof(float), c->stream));
+  HIPCALL(c, hipMemsetAsync(c->d_mus, 0, ((size_t)kMusSlots * 3 * n + kSpare) * sizeof(float), c->stream));
+  c->mus_cur = c->mus_last = 0;
   if (c->d_G) HIPCALL(c, hipMemsetAsync(c->d_G, 0, (kNG * n + kSpare) * sizeof(double), c->stream));
   return FDMOD_OK;
 }
@@ -721,6 +741,7 @@ int grad_backward_next(GradRun &r)
 void swap_grad_mem(fdmod_ctx *c, GradMem &m)
 {
   std::swap(c->d_mu, m.mu);
+  std::swap(c->d_mus, m.mus);
   std::swap(c->d_G, m.G);
   std::swap(c->d_snap, m.snap);
   std::swap(c->d_mu_n, m.n);
@@ -731,7 +752,7 @@ void swap_grad_mem(fdmod_ctx *c, GradMem &m)
 size_t held_bytes(const GradMem &m, size_t n)
 {
   size_t b = m.ckpt.cap_data + m.ckpt.cap_scale + m.hist.cap_data + m.hist.cap_scale;
-  if (m.mu) b += n * (kMaxState * 4 + kNG * 8 + kImgK * 6 * 4);
+  if (m.mu) b += n * (kMaxState * 4 + kMusSlots * 3 * 4 + kNG * 8 + kImgK * 6 * 4);
   return b;
 }
 
@@ -741,7 +762,7 @@ size_t grad_bytes(const fdmod_ctx *c, const fdmod_grad_opts &o)
   const size_t n = field_floats(c), nt = c->p.nt;
   const size_t esz = o.storage == FDMOD_STORE_F32 ? 4 : 2, cksz = o.storage == FDMOD_STORE_F16_ALL ? 2 : 4;
   const int form = c->p.form;
-  size_t b = n * (kMaxState * 4 + kNG * 8 + kImgK * 6 * 4);
+  size_t b = n * (kMaxState * 4 + kMusSlots * 3 * 4 + kNG * 8 + kImgK * 6 * 4);
   Recon rg;
   if (o.history >= FDMOD_HIST_RECONSTRUCT && recon_shape(c, rg, o.history == FDMOD_HIST_RECONSTRUCT_INTERIOR)) {
     const size_t nanch = o.ncheckpoints > 0 ? std::min<size_t>(o.ncheckpoints, nt) : 0;

@@ -145,7 +145,8 @@ struct AdjArgs {
   const float *coef1, *coef2, *invrho;
   const float *c11, *c13, *c15, *c33, *c35, *c55;
   const float *sin2, *cos2, *sin2t, *sn, *cs, *snv, *csv;
-  float *snap[6];                      // V-adjoint: imaging snapshots (or null)
+  float *oxx, *ozz, *oxz;              // V-adjoint: where the updated mu_sigma goes (mxx... in place)
+  float *snap[6];                      // V-adjoint: imaging snapshots of computed fields, [3..] (or null)
   double *G[8];                        // imaging sums (per-form meaning, kernels_adjoint.hip)
 };
 

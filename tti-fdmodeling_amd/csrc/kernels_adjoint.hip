@@ -336,7 +336,7 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
       p.K2 = inU ? a.cos2[o] : 0.0f;
       p.S2T = inU ? a.sin2t[o] : 0.0f;
     }
-    if (FORM == FDMOD_FORM_COORD && a.snap[0] && inU) {
+    if (FORM == FDMOD_FORM_COORD && a.snap[3] && inU) {
       const float kr = a.coef2[o] * a.invrho[o];
       p.na = kr * a.mvx[o];
       p.nb = kr * a.mvz[o];
@@ -389,27 +389,22 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
           a.mtxx[o] = op.k1 * a1;
           a.mtzz[o] = op.k1 * a2;
           a.mtxz[o] = op.k1 * a3;
-          a.mxx[o] = -gxx;
-          a.mzz[o] = -gzz;
-          a.mxz[o] = -gxz;
-          if (a.snap[0]) {
+          a.oxx[o] = -gxx;
+          a.ozz[o] = -gzz;
+          a.oxz[o] = -gxz;
+          if (a.snap[3]) {
             a.snap[3][o] = a1;
             a.snap[4][o] = a2;
             a.snap[5][o] = a3;
           }
         } else {
-          a.mxx[o] = op.k1 * op.s1 - gxx;
-          a.mzz[o] = op.k1 * op.s2 - gzz;
-          a.mxz[o] = op.k1 * op.s3 - gxz;
-          if (FORM == FDMOD_FORM_COORD && a.snap[0] && inU) {
+          a.oxx[o] = op.k1 * op.s1 - gxx;
+          a.ozz[o] = op.k1 * op.s2 - gzz;
+          a.oxz[o] = op.k1 * op.s3 - gxz;
+          if (FORM == FDMOD_FORM_COORD && a.snap[3] && inU) {
             a.snap[3][o] = op.na;
             a.snap[4][o] = op.nb;
           }
-        }
-        if (a.snap[0] && inU) {  // mu_sigma as imaged at this step (after FS^T)
-          a.snap[0][o] = op.s1;
-          a.snap[1][o] = op.s2;
-          a.snap[2][o] = op.s3;
         }
       }
 #pragma unroll
