@@ -148,7 +148,7 @@ SweepArgs with_hist(fdmod_ctx *c, const SweepArgs &sa, size_t slot, bool f16)
 {
   SweepArgs r = sa;
   const int nh = n_hist(c->p.form);
-  const size_t n = field_floats(c), ns = scales_per_field(c);
+  const size_t ns = scales_per_field(c);
   r.estore = f16 ? 2 : 1;
   for (int f = 0; f < nh; f++) {
     if (f16) r.h16[f] = fld(c, (_Float16 *)c->hist.data, slot * nh + f);
@@ -163,7 +163,7 @@ SweepArgs with_hist(fdmod_ctx *c, const SweepArgs &sa, size_t slot, bool f16)
 void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
 {
   const int nh = n_hist(c->p.form);
-  const size_t n = field_floats(c), ns = scales_per_field(c);
+  const size_t ns = scales_per_field(c);
   for (int q = 0; q < 6; q++) b.lam[j][q] = q < 3 ? mus_field(c, c->mus_last, q) : fld(c, c->d_snap, (size_t)j * 6 + q);
   for (int f = 0; f < 8; f++) {
     b.h32[j][f] = nullptr;
@@ -215,7 +215,6 @@ AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
     a.g.izStart -= kN;
     a.g.tilesZ = (a.g.izEnd - a.g.izStart + TZ - 1) / TZ;
   }
-  const size_t n = field_floats(c);
   float *const *d = c->d.a;
   a.form = c->p.form;
   a.mvx = fld(c, c->d_mu, 0); a.mvz = fld(c, c->d_mu, 1);

@@ -406,6 +406,7 @@ __global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
   else ssweep_tile<FORM, PD, ESTORE, false>(a, btz, btx, ring);
 }
 
+// Time index of loop iteration k = *step + off (BT:651): it = k or nt-1-k.
 __device__ __forceinline__ int step_it(const int *step, int off, int nt, int mode)
 {
   const int k = *step + off;
@@ -565,8 +566,6 @@ void launch_source(hipStream_t s, const SourceArgs &a)
   dim3 grid(a.sequential ? 1 : a.ns);
   hipLaunchKernelGGL(k_source, grid, dim3(128), 0, s, a);
 }
-
-// Time index of loop iteration k = *step + off (BT:651): it = k or nt-1-k.
 
 // Batched sources: blockIdx.y = shot; shots without a source in this phase have ns = 0.
 __global__ void k_source_batch(const SourceArgs *__restrict__ shots, const int *step, int off, int nt, int mode)
