@@ -423,11 +423,6 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
 // scale per field, column and 64-row block: a block with maximum |v| < 2^E is stored
 // as v * 2^(14-E) (exact scaling, |value| < 2^14 < 65504) and the block keeps the
 // inverse scale 2^(E-14).  Arithmetic on the decoded values stays in FP32.
-__device__ __forceinline__ float wave_max(float m)
-{
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-  return m;
-}
 
 // One wave per (column, block); blockIdx.y = field.  Rows [Nz, ldz) hold zeros.
 __global__ __launch_bounds__(256) void k_pack16(Pack16 p)
@@ -440,7 +435,7 @@ __global__ __launch_bounds__(256) void k_pack16(Pack16 p)
   const int ix = (int)(w / nblk), b = (int)(w - (long)ix * nblk);
   const size_t o = (size_t)ix * p.ldz + (size_t)b * TZ + lane;
   const float v = p.src[f][o];
-  const float m = wave_max(fabsf(v));
+  const float m = wave_max64(fabsf(v));  // every lane of the wave is active here
   int E = 0;
   if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
   E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats

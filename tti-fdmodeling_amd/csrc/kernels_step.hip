@@ -37,7 +37,7 @@ __device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, 
   float m = 0.0f;
 #pragma unroll
   for (int f = 0; f < NF; f++) m = fmaxf(m, fabsf(e[f]));
-  for (int s = 32; s > 0; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+  m = wave_max64(m);
   int E = 0;
   if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
   E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats

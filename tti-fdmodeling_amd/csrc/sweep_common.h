@@ -21,6 +21,29 @@ __device__ __forceinline__ float ldnt(const float *p) { return __builtin_nontemp
 __device__ __forceinline__ float ldst(const float *p) { return *p; }
 __device__ __forceinline__ void st_out(float *p, float v) { *p = v; }
 
+// Maximum of v over the 64 lanes of the wave, the same in every lane.  DPP moves:
+// within each row of 16 lanes a butterfly (quad_perm [1,0,3,2] and [2,3,0,1], then
+// row_ror 4 and 8), then row_bcast 15 into rows 1 and 3 and row_bcast 31 into rows 2
+// and 3, so lane 63 holds the wave's maximum; lanes a move leaves out keep their own
+// value.  fmaxf is exact, so the order of the reduction does not matter.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_max(float v)
+{
+  const int w = __float_as_int(v);
+  return fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(w, w, CTRL, ROW_MASK, 0xF, false)));
+}
+
+__device__ __forceinline__ float wave_max64(float v)
+{
+  v = dpp_max<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v = dpp_max<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v = dpp_max<0x124, 0xF>(v);  // row_ror 4
+  v = dpp_max<0x128, 0xF>(v);  // row_ror 8
+  v = dpp_max<0x142, 0xA>(v);  // row_bcast 15 -> rows 1, 3
+  v = dpp_max<0x143, 0xC>(v);  // row_bcast 31 -> rows 2, 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // AND of the fine "undamped" flags covering columns [xa, xa+nc) of strip tz.
 __device__ __forceinline__ bool segment_undamped(const Grid &g, const uint8_t *flags, int tz, int xa, int nc)
 {
