@@ -28,7 +28,8 @@ constexpr double kPI = 3.141592653589793;
 constexpr int TZ = 64;
 constexpr int kRing = 11;          // LDS ring of staged columns (10 live + 1 in flight)
 constexpr int kFlagW = 11;         // columns per fine flag tile
-constexpr int kDefaultSegLV = 44;  // measured best on a 1256 x 8002 grid
+constexpr int kDefaultSegLV = 44;  // longest velocity-sweep segment
+constexpr int kVBlocksPerCU = 16;  // velocity-sweep blocks per CU: 160 KB LDS / 9.6 KB ring
 // Stress-sweep segments: 11 columns on every grid.  Odd segments walk right to left
 // and share their halo through L2.  Measured on 1256 x 8002 with 128-byte-aligned
 // strips (us per launch, segments of 4 / 6 / 8 / 11 / 16 columns): Bond 99.7 / 95.3 /
@@ -37,7 +38,6 @@ constexpr int kDefaultSegLV = 44;  // measured best on a 1256 x 8002 grid
 constexpr int kDefaultSegLS = 11;
 inline int default_segLS(int /*form*/, int /*cols*/) { return kDefaultSegLS; }
 constexpr int kZAlign = 32;    // floats per 128-byte line: strips start on a line
-constexpr int kMinTiles = 3000;    // blocks per sweep below which segments shorten
 // Padded grid points per shot group of a batched loop: about one full-size shot
 // (10240 columns of 1280 rows); measured on 64 crops 1126 deep, groups of 9-10k
 // columns ran 8-10% faster than one group (48.3 vs 44.5 Gpts/s).
