@@ -154,7 +154,7 @@ struct AdjArgs {
 // (Bond 3, tensor 6, coordinate 5 fields) and the forward history of the step
 // (Bond 3, tensor 6, coordinate 8 fields; FP32 h32, or FP16 h16 with the per-(column,
 // strip) scales hsc of its two field groups).
-constexpr int kImgK = 8;
+constexpr int kImgK = 16;
 struct ImgBlock {
   int nk;
   const float *lam[kImgK][6];

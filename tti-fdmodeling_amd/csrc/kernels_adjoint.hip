@@ -497,9 +497,10 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
     P[0] = a.c11[o]; P[1] = a.c13[o]; P[2] = a.c33[o]; P[3] = a.c55[o]; P[4] = a.coef2[o];
     P[5] = a.sn[o]; P[6] = a.cs[o];
   }
-#pragma unroll
-  for (int j = 0; j < kImgK; j++) {
-    if (j >= b.nk) break;
+  // fully unrolled (compile-time j): the block's pointers then come from fixed kernel-argument offsets
+  static_for<0, kImgK>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j >= b.nk) return;
     double L[NL], H[NH];
 #pragma unroll
     for (int q = 0; q < NL; q++) L[q] = b.lam[j][q][o];
@@ -554,7 +555,7 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
       acc[6] += L[3] * H[4] + L[4] * H[5];  // sinv: a (-sxxdz + sxzdx) + b (-sxzdz + szzdx)
       acc[7] += L[3] * H[6] + L[4] * H[7];  // cosv: a (sxxdx + sxzdz) + b (sxzdx + szzdz)
     }
-  }
+  });
 #pragma unroll
   for (int q = 0; q < NG; q++) a.G[q][o] += acc[q];
 }
