@@ -507,6 +507,10 @@ def main():
                    "grid_padded": [g.Nz, g.Nx], "updated_points_per_step": pts, "parallelism": "single GPU"},
         "device_seconds": round(dev_s, 4),
         "traces_finite": finite,
+        "note": ("the reference's Bond free-surface fix-up (BT:543-568) divides 0/0 in water, so this marine "
+                 "model's Bond traces are NaN in the reference too and here bit for bit; forms_gpts gives the "
+                 "coordinate and tensor forms, whose traces are finite") if args.form == "bond" and not finite
+                else None,
         "roofline": roofline,
         "forms_gpts": forms,
         "batched": batched,
