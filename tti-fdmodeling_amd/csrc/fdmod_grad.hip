@@ -295,7 +295,9 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *src
                       srcadj);
   }
   if (p.freesurface) launch_adj_fs(s, a);
-  launch_adj_ssweep(s, a);
+  AdjArgs as = a;  // the stress adjoint's strips output kTZS rows each
+  as.g.tilesZ = (c->g.izEnd - c->g.izStart + kTZS - 1) / kTZS;
+  launch_adj_ssweep(s, as);
   if (rec && ds.vx)
     launch_adj_inject(s, c->nr, c->g.ldz, c->d_gx, c->d_gz, ds.vx + so, ds.vz + so, a.mvx, a.mvz, 1.0f);
   if (srcadj && c->ns > 0 && c->stype != 1) {

@@ -37,9 +37,6 @@ constexpr int kRowMax = 14;
 struct MRow {
   float v[kRowMax];
 };
-struct MCol {
-  MRow lo, hi;
-};
 
 template <int FORM>
 __device__ __forceinline__ void load_mrow(MRow &r, const AdjArgs &a, size_t o)
@@ -55,17 +52,6 @@ __device__ __forceinline__ void load_mrow(MRow &r, const AdjArgs &a, size_t o)
   } else {
     r.v[5] = a.c33[o]; r.v[6] = a.c55[o]; r.v[7] = a.coef2[o]; r.v[8] = a.sn[o]; r.v[9] = a.cs[o];
   }
-}
-
-template <int FORM>
-__device__ __forceinline__ void load_mcol(MCol &c, const AdjArgs &a, int gx, int rs, int lane, const Grid &g)
-{
-  const size_t base = (size_t)gx * g.ldz;
-  const int r0 = rs + lane, r1 = rs + TZ + lane;
-  c.lo = MRow{};
-  c.hi = MRow{};
-  if (r0 >= 0 && r0 < g.Nz) load_mrow<FORM>(c.lo, a, base + r0);
-  if (lane < kHalo && r1 >= 0 && r1 < g.Nz) load_mrow<FORM>(c.hi, a, base + r1);
 }
 
 template <int FORM>
@@ -108,21 +94,6 @@ __device__ __forceinline__ void m_of(const MRow &r, float (&m)[4])
   }
 }
 
-template <int FORM>
-__device__ __forceinline__ void store_mcol(float (*ring)[kRing][LZ], int slot, const MCol &c, int lane)
-{
-  constexpr int NM = m_fields<FORM>();
-  float m[4];
-  m_of<FORM>(c.lo, m);
-#pragma unroll
-  for (int f = 0; f < NM; f++) ring[f][slot][lane] = m[f];
-  if (lane < kHalo) {
-    m_of<FORM>(c.hi, m);
-#pragma unroll
-    for (int f = 0; f < NM; f++) ring[f][slot][TZ + lane] = m[f];
-  }
-}
-
 struct VAdjOps {
   float mvx, mvz, k1;
 };
@@ -130,30 +101,52 @@ struct VAdjOps {
 // mu_v(p) <- coef1(p) mu_v(p) - (adjoint of the gradients of v read by the stress
 // sweep), p in U.  Bond/tensor: (Dx M0 + Dz M2, Dx M2 + Dz M1); coordinate:
 // (Dx M0 + Dz M1, Dx M2 + Dz M3).
+// Staged column of the stress adjoint: lane l holds row rs + l only (the strip's 64
+// staged rows are its kTZS = 55 output rows and their 4 + 5 halo rows).
+template <int FORM>
+__device__ __forceinline__ void load_mrow_lane(MRow &r, const AdjArgs &a, int gx, int row, const Grid &g)
+{
+  r = MRow{};
+  if (row >= 0 && row < g.Nz) load_mrow<FORM>(r, a, (size_t)gx * g.ldz + row);
+}
+
+template <int FORM>
+__device__ __forceinline__ void store_mrow(float (*ring)[kRing][TZ], int slot, const MRow &r, int lane)
+{
+  constexpr int NM = m_fields<FORM>();
+  float m[4];
+  m_of<FORM>(r, m);
+#pragma unroll
+  for (int f = 0; f < NM; f++) ring[f][slot][lane] = m[f];
+}
+
+// The strip of block row btz outputs rows z0 .. z0+54 (lanes 4..58) and stages rows
+// z0-4 .. z0+59, one per lane: each staged value is loaded and turned into M once,
+// where 64-row outputs need a second, mostly idle load of the 9 halo rows per field.
 template <int FORM, int PD>
 __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
 {
   constexpr int NM = m_fields<FORM>();
-  __shared__ float ring[NM][kRing][LZ];
+  __shared__ float ring[NM][kRing][TZ];
   const Grid &g = a.g;
   const int segL = g.segLV;
   const int lane = threadIdx.x;
-  const int z0 = g.izStart + btz * TZ;
+  const int z0 = g.izStart + btz * kTZS;
   const int xa = g.ixStart + btx * segL;
   const int nc = min(segL, g.ixEnd - xa);
-  const int iz = z0 + lane;
-  const bool zok = iz < g.izEnd;
   const int rs = z0 - 4;
+  const int iz = rs + lane;
+  const bool zok = lane >= 4 && lane < 4 + kTZS && iz < g.izEnd;
   // prologue in groups of three columns (a staged row is up to 14 floats)
 #pragma unroll
   for (int m0 = 0; m0 < 9; m0 += 3) {
-    MCol c[3];
+    MRow c[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) load_mcol<FORM>(c[m], a, xa - 4 + m0 + m, rs, lane, g);
+    for (int m = 0; m < 3; m++) load_mrow_lane<FORM>(c[m], a, xa - 4 + m0 + m, iz, g);
 #pragma unroll
-    for (int m = 0; m < 3; m++) store_mcol<FORM>(ring, m0 + m, c[m], lane);
+    for (int m = 0; m < 3; m++) store_mrow<FORM>(ring, m0 + m, c[m], lane);
   }
-  MCol colq[PD];
+  MRow colq[PD];
   VAdjOps opq[PD];
   auto load_ops = [&](VAdjOps &p, int col) {
     const size_t o = (size_t)col * g.ldz + iz;
@@ -163,7 +156,7 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   };
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    if (i <= nc - 1) load_mcol<FORM>(colq[i], a, xa + 5 + i, rs, lane, g);
+    if (i <= nc - 1) load_mrow_lane<FORM>(colq[i], a, xa + 5 + i, iz, g);
     if (zok && i < nc) load_ops(opq[i], xa + i);
   }
   const float invdx = g.invdx, invdz = g.invdz;
@@ -173,19 +166,19 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
       const int k = k0 + u;
       if (k >= nc) return;
       const int c = xa + k;
-      MCol cn;
+      MRow cn;
       VAdjOps on;
-      if (k <= nc - PD - 1) load_mcol<FORM>(cn, a, c + 5 + PD, rs, lane, g);
+      if (k <= nc - PD - 1) load_mrow_lane<FORM>(cn, a, c + 5 + PD, iz, g);
       if (zok && k + PD < nc) load_ops(on, c + PD);
-      store_mcol<FORM>(ring, (u + 9) % kRing, colq[0], lane);
+      store_mrow<FORM>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
       if (zok) {
         float d1, d2;
-        rsg_fwd([&](int dx, int dz) { return ring[0][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        rsg_fwd([&](int dx, int dz) { return ring[0][(u + dx + 4 + kRing) % kRing][lane + dz]; }, d1, d2);
         const float m0dx = 0.5f * (d1 + d2) * invdx;
-        rsg_fwd([&](int dx, int dz) { return ring[1][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        rsg_fwd([&](int dx, int dz) { return ring[1][(u + dx + 4 + kRing) % kRing][lane + dz]; }, d1, d2);
         const float m1dz = 0.5f * (d1 - d2) * invdz;
-        rsg_fwd([&](int dx, int dz) { return ring[2][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        rsg_fwd([&](int dx, int dz) { return ring[2][(u + dx + 4 + kRing) % kRing][lane + dz]; }, d1, d2);
         const float m2dx = 0.5f * (d1 + d2) * invdx;
         const float m2dz = 0.5f * (d1 - d2) * invdz;
         const size_t o = (size_t)c * g.ldz + iz;
@@ -194,8 +187,7 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
           a.mvx[o] = op.k1 * op.mvx - (m0dx + m2dz);
           a.mvz[o] = op.k1 * op.mvz - (m2dx + m1dz);
         } else {
-          rsg_fwd([&](int dx, int dz) { return ring[NM - 1][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1,
-                  d2);
+          rsg_fwd([&](int dx, int dz) { return ring[NM - 1][(u + dx + 4 + kRing) % kRing][lane + dz]; }, d1, d2);
           const float m3dz = 0.5f * (d1 - d2) * invdz;
           a.mvx[o] = op.k1 * op.mvx - (m0dx + m1dz);
           a.mvz[o] = op.k1 * op.mvz - (m2dx + m3dz);
