@@ -327,7 +327,7 @@ def cpu_gradient_replica(seed=7):
     nz, W, nt = 141, 250, 800
     full = synth.model_large(nz=nz, nx=W, seed=seed)
     # a change the data see within nt steps: +3% vp in the rock, plus two anomalies
-    true = synth.add_anomalies(full, n=2)
+    true = synth.add_anomalies(full, n=2, zrange=(0.35, 0.85))
     true["vp"] = np.where(full["vs"] > 0, true["vp"] * 1.03, true["vp"]).astype(np.float32)
     kw = dict(ealThick=30, freesurface=0, fpeak=12.0)
 

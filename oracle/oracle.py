@@ -102,10 +102,13 @@ class Oracle:
         self.nr = 0
         self.ntsample = 0
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "c", None):
             self.L["destroy"](self.c)
             self.c = None
+
+    def __del__(self):
+        self.close()
 
     def set_model(self, vp, vs, rho, eps, delta, theta):
         arrs = [_f32(a).reshape(-1) for a in (vp, vs, rho, eps, delta, theta)]

@@ -77,3 +77,25 @@ def test_mode2_mt_bitwise():
     g.run()
     for comp in (1, 3, 4):
         assert np.array_equal(g.traces(comp), o.traces(comp).astype(np.float32)), comp
+
+
+# dx != dz: every invdx / invdz placement, the MEAL thicknesses in x and z, the source
+# weights (rx, rz) and the coordinate form's invdx on d(sxx)/dz (CR:633) differ from
+# their swapped forms only when the spacings differ
+@pytest.mark.parametrize("form", ["bond", "tensor", "coord"])
+@pytest.mark.parametrize("fs,stype", [(0, 1), (1, 1), (1, 4), (0, 7)])
+def test_traces_bitwise_dx_ne_dz(form, fs, stype):
+    water = 3 if fs and form != "bond" else 0  # water under the Bond fix-up is NaN (BT:543-568)
+    o, g, _ = _pair(form=form, freesurface=fs, stype=stype, nt=300, dx=8.0, dz=12.5, water=water)
+    for name in PARAMS[form]:
+        a, b = g.get(name), o.get(name).astype(np.float32)
+        assert np.array_equal(a, b, equal_nan=True), name
+    o.run()
+    g.run()
+    for comp in (1, 3, 4):
+        a, b = g.traces(comp), o.traces(comp).astype(np.float32)
+        assert np.isfinite(a).all()
+        assert np.array_equal(a, b), f"comp {comp}: rel l2 {rel_l2(a, b):.3e}"
+    fields = ("vx", "vz", "sxx", "szz", "sxz") + (("txx", "tzz", "txz") if form == "tensor" else ())
+    for f in fields:
+        assert np.array_equal(g.get(f), o.get(f).astype(np.float32)), f

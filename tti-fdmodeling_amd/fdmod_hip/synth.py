@@ -116,9 +116,14 @@ def model_large(nz=1126, nx=7872, dx=10.0, seed=2, water=300.0):
     return dict(vp=f32(vp), vs=f32(vs), rho=f32(rho), eps=f32(eps), delta=f32(dlt), theta=f32(th))
 
 
-def add_anomalies(model, dx=10.0, n=5, sigma=150.0, seed=4, dvp=0.08, deps=0.05):
+def add_anomalies(model, dx=10.0, n=5, sigma=150.0, seed=4, dvp=0.08, deps=0.05, xrange=(0.15, 0.85),
+                  zrange=(0.07, 0.25)):
     """True model for gradient tests: background + n Gaussian anomalies (sigma in m,
-    +-dvp relative in vp, +-deps absolute in epsilon), placed below the water layer."""
+    +-dvp relative in vp, +-deps absolute in epsilon) in the rock below the water layer,
+    centred at seeded positions within the fractions xrange of the width and zrange of
+    the depth.  The default depths (0.8-2.8 km on the 1126-deep model) are ones whose
+    reflections reach the surface receivers within nt = 4000 steps of 0.8 ms; anomalies
+    deeper than about 3.4 km leave those records unchanged to FP32 rounding."""
     rng = np.random.default_rng(seed)
     nx, nz = model["vp"].shape
     X = np.arange(nx)[:, None] * dx
@@ -127,8 +132,8 @@ def add_anomalies(model, dx=10.0, n=5, sigma=150.0, seed=4, dvp=0.08, deps=0.05)
     g_ep = np.zeros((nx, nz))
     rock = model["vs"] > 0
     for _ in range(n):
-        cx = rng.uniform(0.15, 0.85) * (nx - 1) * dx
-        cz = rng.uniform(0.35, 0.85) * (nz - 1) * dx
+        cx = rng.uniform(*xrange) * (nx - 1) * dx
+        cz = rng.uniform(*zrange) * (nz - 1) * dx
         g = np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * sigma ** 2))
         g_vp += rng.choice([-1.0, 1.0]) * dvp * g
         g_ep += rng.choice([-1.0, 1.0]) * deps * g
