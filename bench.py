@@ -37,6 +37,9 @@ import time
 # cores); set before any library in this process starts an OpenMP runtime.
 os.environ.setdefault("OMP_PROC_BIND", "spread")
 os.environ.setdefault("OMP_PLACES", "cores")
+# The CPUs this process may use, read before an OpenMP runtime binds the main thread to
+# its first place (after which the thread's own mask holds one CPU).
+_CPUS = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
 
 import numpy as np
 
@@ -115,7 +118,7 @@ def host_cpu():
                     break
     except OSError:
         pass
-    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cpus = _CPUS
     cores, sockets = set(), set()
     for c in cpus:
         try:
