@@ -87,6 +87,30 @@ def test_meal_profile(ealThick, R):
     assert np.all(c1[n:-n, n:-n] == 1.0) and np.all(c2[n:-n, n:-n] == np.float64(np.float32(1e-3)))
 
 
+@pytest.mark.parametrize("eal_alpha,alpha", [(-1.0, 2.5), (1.5, 1.5)])
+def test_meal_inner_layers_closed_form(eal_alpha, alpha):
+    """Every layer l = 1..n of the profile (meal.c:23-29, 74-77; n = ealThick + 5): on a side
+    column at mid depth D*Beta = d0x w(l) and Beta - 1 = |beta_x - 1| w(l), on the top row
+    at mid width the same with dz, where w(l) = 2^((l/n)^alpha) - 1 (alpha = 2.5 for the
+    default eal_alpha = -1), d0 = 3 vp ln(1/R) / (2 n h), beta = 2 vs / (h fpeak) / 10.
+    Layer l of the left side is column n - l, of the top row is row n - l."""
+    nz, nx, dx, dz, vpv, vsv, fpeak = 24, 26, 8.0, 12.5, 2500.0, 1200.0, 15.0
+    ones = np.ones((nx, nz), np.float32)
+    o = Oracle("bond", nz, nx, 4, dx, dz, 1e-3, ealThick=15, eal_alpha=eal_alpha, fpeak=fpeak, prec="f64")
+    o.set_model(vpv * ones, vsv * ones, 2000 * ones, 0 * ones, 0 * ones, 0 * ones)
+    n = o.e
+    assert n == 20
+    R = 1e-5  # 10^-((log10 20 - 1)/log10 2 + 3 + 1)
+    D, B = o.get("D"), o.get("Beta")
+    for l in range(1, n + 1):
+        w = 2.0 ** ((l / n) ** alpha) - 1.0
+        for h, (ix, iz) in ((dx, (n - l, o.Nz // 2)), (dz, (o.Nx // 2, n - l))):
+            d0 = 3.0 * vpv * np.log(1.0 / R) / (2 * n * h)
+            beta = 2.0 * vsv / (h * fpeak) / 10.0
+            assert abs(D[ix, iz] * B[ix, iz] - d0 * w) <= 1e-9 * d0, (l, h)
+            assert abs(B[ix, iz] - 1.0 - abs(beta - 1.0) * w) <= 1e-9, (l, h)
+
+
 def test_meal_free_surface_leaves_top_undamped():
     nz, nx = 12, 14
     ones = np.ones((nx, nz), np.float32)
@@ -320,6 +344,33 @@ def test_source_weights_hand_values():
     assert np.count_nonzero(vx) == 121
 
 
+@pytest.mark.parametrize("stype", [5, 6, 7])
+def test_force_sources_hand_values_dx_ne_dz(stype):
+    """stype 5/6/7 (BT:740-812): w = exp(alpha (rx^2 + rz^2)) dt/(dx dz) with
+    alpha = ln(0.8)/(dx^2 + dz^2), so with dx = 8, dz = 12.5 the diagonal neighbour gets
+    0.8 w0 and the axis neighbours 0.8^(dx^2/(dx^2+dz^2)) w0 (x) and 0.8^(dz^2/(dx^2+dz^2))
+    w0 (z); 5 loads vx, 6 loads vz, 7 loads both with sqrt(2)/2."""
+    dx, dz = 8.0, 12.5
+    ctor, m, geom = small_case(nz=30, nx=30, nt=4, stype=stype, ealThick=2, dx=dx, dz=dz)
+    geom["src"] = np.full_like(geom["src"], 3.0)
+    o = run_case(Oracle, ctor, m, geom, prec="f64")
+    o.sources(0, 0)
+    vx, vz = o.get("vx"), o.get("vz")
+    ix, iz = geom["spx"][0] + o.e, geom["spz"][0] + o.e
+    w0 = 3.0 * float(np.float32(1e-3)) / (dx * dz)
+    q = float(np.float32(0.8))  # logf(0.8f), BT:647
+    gx, gz = q ** (dx * dx / (dx * dx + dz * dz)), q ** (dz * dz / (dx * dx + dz * dz))
+    amp = {5: (1.0, 0.0), 6: (0.0, 1.0), 7: (np.sqrt(0.5), np.sqrt(0.5))}[stype]
+    for f, a in ((vx, amp[0]), (vz, amp[1])):
+        if a == 0.0:
+            assert np.count_nonzero(f) == 0
+            continue
+        assert np.count_nonzero(f) == 121
+        for (ox, oz), fac in (((0, 0), 1.0), ((1, 1), q), ((-1, 1), q), ((1, 0), gx), ((0, -1), gz),
+                              ((5, 5), q ** 25), ((-5, 2), gx ** 25 * gz ** 4)):
+            assert abs(f[ix + ox, iz + oz] - a * fac * w0) <= 1e-10 * w0, (stype, ox, oz)
+
+
 def test_source_curl_and_divergence_symmetry():
     ctor, m, geom = small_case(nz=30, nx=30, nt=4, stype=2, ealThick=2)
     geom["src"] = np.ones_like(geom["src"])
@@ -466,3 +517,82 @@ def test_acoustic_limit_matches_2d_greens_function():
         amp = np.linalg.norm(num) / np.linalg.norm(ana)
         assert cc > 0.997, (off, cc)
         assert abs(amp - 1) < 0.01, (off, amp)
+
+
+# ---------------------------------------------------------------- energy diagnostic
+def _undamped(o):
+    o.set("coef1", np.ones((o.Nx, o.Nz)))
+    o.set("coef2", np.full((o.Nx, o.Nz), float(np.float32(1e-3))))
+
+
+@pytest.mark.parametrize("state,ek,ep", [
+    # (vx, vz, sxx, szz, sxz), hand values per cell of Ek/(dx dz) and Ep/(dx dz) with
+    # 1/invrho = 2, c11 = 4, c55 = 1: Ek = (vx^2 + vz^2) / invrho / 2; the reference's Ep
+    # (BT:936-946) reduces to ((c11 + 2 c55)(sxx^2 - szz^2) / (4 c55 (c11 + c55)) + sxz^2 / c55) / 2
+    ((1.0, 2.0, 0.0, 0.0, 0.0), 5.0, 0.0),
+    ((0.0, 0.0, 3.0, 0.0, 0.0), 0.0, 1.35),
+    ((0.0, 0.0, 0.0, 1.0, 0.0), 0.0, -0.15),     # szz alone counts negatively, as the reference's sum does
+    ((0.0, 0.0, 0.0, 0.0, 0.5), 0.0, 0.125),
+    ((1.0, 2.0, 3.0, 1.0, 0.5), 5.0, 1.325),     # the sxx*szz cross terms cancel
+])
+def test_energy_hand_values(state, ek, ep):
+    nz, nx, dx, dz = 9, 11, 8.0, 12.5
+    ctor, m, geom = small_case(nz=nz, nx=nx, nt=2, ealThick=2, dx=dx, dz=dz)
+    geom["src"] = np.zeros_like(geom["src"])
+    o = run_case(Oracle, ctor, m, geom, prec="f64")
+    o.set_energy(1)
+    _undamped(o)
+    full = lambda v: np.full((o.Nx, o.Nz), v)
+    o.set("invrho", full(0.5))
+    o.set("c11", full(4.0))
+    o.set("c55", full(1.0))
+    for name, v in zip(("vx", "vz", "sxx", "szz", "sxz"), state):
+        o.set(name, full(v))
+    o.run(0, 1)  # a constant state has zero derivatives: one undamped step leaves it as it is
+    E, Ek, Ep = o.energy()
+    cells = nx * nz * dx * dz
+    assert abs(Ek[0] - ek * cells) <= 1e-12 * cells
+    assert abs(Ep[0] - ep * cells) <= 1e-12 * cells
+    assert abs(E[0] - (ek + ep) * cells) <= 1e-12 * cells
+
+
+def test_energy_kinetic_part_bounded_by_the_conserved_energy():
+    """No damping (coef1 = 1, coef2 = dt everywhere), no source, a smooth initial pulse in
+    an isotropic medium: the kinetic energy the diagnostic reports oscillates below the
+    initial total elastic energy (computed here from the fields with the isotropic
+    compliance), and the wavefield keeps that total to a few percent."""
+    nz, nx, nt = 40, 44, 300
+    ctor, m, geom = small_case(nz=nz, nx=nx, nt=nt, ealThick=2)
+    m = {k: v.copy() for k, v in m.items()}
+    m["eps"][:] = 0.0
+    m["delta"][:] = 0.0
+    m["theta"][:] = 0.0
+    geom["src"] = np.zeros_like(geom["src"])
+    o = run_case(Oracle, ctor, m, geom, prec="f64")
+    o.set_energy(1)
+    _undamped(o)
+    X, Z = np.meshgrid(np.arange(o.Nx), np.arange(o.Nz), indexing="ij")
+    bump = np.exp(-((X - o.Nx / 2) ** 2 + (Z - o.Nz / 2) ** 2) / (2 * 4.0 ** 2))
+    o.set("sxx", 1e3 * bump)
+    o.set("szz", 4e2 * bump)
+    o.set("vx", 1e-4 * np.roll(bump, 3, axis=0))
+    # the total over the whole grid: without damping the waves cross into the padding,
+    # which the diagnostic (a sum over the model region) leaves out
+    sl = (slice(None), slice(None))
+    c11, c13, c55 = o.get("c11")[sl], o.get("c13")[sl], o.get("c55")[sl]
+    rho = 1.0 / o.get("invrho")[sl]
+
+    def total():
+        vx, vz = o.get("vx")[sl], o.get("vz")[sl]
+        a, b, c = o.get("sxx")[sl], o.get("szz")[sl], o.get("sxz")[sl]
+        w = (c11 * a * a - 2 * c13 * a * b + c11 * b * b) / (c11 * c11 - c13 * c13) + c * c / c55
+        return 0.5 * float((rho * (vx * vx + vz * vz) + w).sum()) * 100.0
+
+    E0 = total()
+    for k in range(nt):
+        o.run(k, 1)
+        if k % 50 == 49:
+            assert abs(total() / E0 - 1.0) < 0.05, k
+    _, Ek, _ = o.energy()
+    assert np.all(Ek >= 0) and Ek.max() <= 1.02 * E0 and Ek.max() >= 0.3 * E0
+    assert np.abs(np.diff(np.sign(np.diff(Ek)))).sum() > 2  # it oscillates, it does not just decay
