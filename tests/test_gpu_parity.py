@@ -99,3 +99,29 @@ def test_traces_bitwise_dx_ne_dz(form, fs, stype):
     fields = ("vx", "vz", "sxx", "szz", "sxz") + (("txx", "tzz", "txz") if form == "tensor" else ())
     for f in fields:
         assert np.array_equal(g.get(f), o.get(f).astype(np.float32)), f
+
+
+# snapshots (isoelastic.c:240-289): savew = 1 keeps the absorbing layers (the whole
+# padded grid), savew = 0 crops them; vx and vz as they are, p = 0.5 (sxx + szz)
+@pytest.mark.parametrize("savew", [0, 1])
+def test_snapshots_bitwise(savew):
+    nt, every = 120, 40
+    ctor, m, geom = small_case(nz=30, nx=40, nt=nt, form="coord", freesurface=1)
+    g = run_case(Modeling, ctor, m, geom)
+    snaps = {}
+    g.set_snapshots(3, savew, every, lambda it, kind, a: snaps.__setitem__((it, kind), a))
+    g.run()
+    o = run_case(Oracle, ctor, m, geom)
+    e = 0 if savew else o.e
+    crop = (slice(e, o.Nx - e), slice(e, o.Nz - e))
+    for k in range(nt):
+        o.run(k, 1)
+        if k % every:
+            continue
+        vx, vz = o.get("vx").astype(np.float32), o.get("vz").astype(np.float32)
+        p = np.float32(0.5) * (o.get("sxx").astype(np.float32) + o.get("szz").astype(np.float32))
+        for kind, ref in (("vx", vx), ("vz", vz), ("p", p)):
+            got = snaps[(k, kind)]
+            assert got.shape == ref[crop].shape, (k, kind, got.shape)
+            assert np.array_equal(got, ref[crop]), (k, kind)
+    assert sorted({it for it, _ in snaps}) == list(range(0, nt, every))

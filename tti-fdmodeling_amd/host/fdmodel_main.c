@@ -209,42 +209,107 @@ int fdmodel_main(int argc, char **argv, int form, const char **sdoc)
   char *vpfile = NULL, *vsfile = NULL, *rhofile = NULL, *thetafile = NULL, *epsilonfile = NULL, *deltafile = NULL;
   char *sourcefile = NULL, *recposfile = NULL, *trace_prefix = NULL, *snap_prefix = NULL;
 
+  /* parameters, defaults and diagnostic lines of the reference programs (BT:134-294,
+   * the same in TR and CR), including their wording */
   if (!getparfloat("dx", &dx)) err("<error>: must specify sampling interval in 1st dimension!");
   if (!getparfloat("dz", &dz)) err("<error>: must specify sampling interval in 2nd dimension!");
   if (!getparfloat("dt", &dt)) err("<error>: must specify sampling interval in time dimension!");
-  if (!getparfloat("fx", &fx)) fx = 0.0f, warn("<warning>: fx not given, using %f.", fx);
-  if (!getparfloat("fz", &fz)) fz = 0.0f, warn("<warning>: fz not given, using %f.", fz);
-  if (!getparfloat("scalar", &scalar)) scalar = 1.0f, warn("<warning>: scalar for theta not given, using %f.", scalar);
+  if (!getparfloat("fx", &fx)) {
+    fx = 0.0f;
+    warn("<warning>: didn't specify first sample in 2nd dimension. The program will set it as %f.", fx);
+  }
+  if (!getparfloat("fz", &fz)) {
+    fz = 0.0f;
+    warn("<warning>: didn't specify first sample in 1st dimension. The program will set it as %f.", fz);
+  }
+  if (!getparfloat("scalar", &scalar)) {
+    scalar = 1.0f;
+    warn("<warning>: didn't specify the scalar to be apllied to dip angle. The program will set it as %f.", scalar);
+  }
   if (!getparfloat("fpeak", &fpeak)) err("<error>: must specify peak frequency of source wavelet!");
-  if (!getparfloat("eal_alpha", &eal_alpha)) eal_alpha = -1.0f, warn("<warning>: eal_alpha not given, using the default.");
+  if (!getparfloat("eal_alpha", &eal_alpha)) {
+    eal_alpha = -1.0f;
+    warn("<warning>: didn't specify the order of gaussian weighting function! The program will set it as 0.05 * "
+         "ealThick + 1.5.");
+  }
   if (!getparint("nz", &nz)) err("<error>: must specify the number of samples in 1st dimension!");
   if (!getparint("nx", &nx)) err("<error>: must specify the number of samples in 2nd dimension!!");
   if (!getparint("nt", &nt)) err("<error>: must specify the number of samples in time dimension!!");
-  if (!getparint("freesurface", &freesurface)) freesurface = 0, warn("<warning>: freesurface not given, using 0.");
-  if (!getparint("ealThick", &ealThick)) ealThick = 30, warn("<warning>: ealThick not given, using %d.", ealThick);
-  if (!getparint("mode", &mode)) mode = 1, warn("<warning>: mode not given, using %d (forward).", mode);
-  if (!getparint("savew", &savew)) savew = 0, warn("<warning>: savew not given, using %d.", savew);
-  if (!getparint("wsave", &wsave)) wsave = 1, warn("<warning>: wsave not given, using %d.", wsave);
-  if (!getparint("stype", &stype)) stype = 1, warn("<warning>: stype not given, using %d.", stype);
-  if (!getparint("seistype", &seistype)) seistype = 0, warn("<warning>: seistype not given, using %d.", seistype);
-  if (!getparint("mt", &mt)) mt = 1, warn("<warning>: mt not given, using %d.", mt);
+  if (!getparint("freesurface", &freesurface)) {
+    freesurface = 0;
+    warn("<warning>: didn't specify whether to use free surface boundary conditions. The program will not use it.");
+  }
+  if (!getparint("ealThick", &ealThick)) {
+    ealThick = 30;
+    warn("<warning>: didn't specify the number of absorbing layers! The program will set it %d.", ealThick);
+  }
+  if (!getparint("mode", &mode)) {
+    mode = 1;
+    warn("<warning>: didn't specify the mode of modelng! The program will it as %d which means the program will "
+         "execute the forward modeling.",
+         mode);
+  }
+  if (!getparint("savew", &savew)) {
+    savew = 0;
+    warn("<warning>: didn't specify whether to save the wavefiled. The program will set it as %d which means the "
+         "program will not save the wavefiled of absoring boundary area.",
+         savew);
+  }
+  if (!getparint("wsave", &wsave)) {
+    wsave = 1;
+    warn("<warning>: didn't specify which wavefiled will be saved. The program will set it as %d which means the "
+         "program will not save the wavefiled.",
+         wsave);
+  }
+  if (!getparint("stype", &stype)) {
+    stype = 1;
+    warn("<warning>: didn't specify the type of source,  and will set stype as %d!", stype);
+  }
+  if (!getparint("seistype", &seistype)) {
+    seistype = 0;
+    /* the reference prints savew's value in this line */
+    warn("<warning>: didn't specify which wavefiled will be recorded. The program will set it as %d which means the "
+         "program will not save the seismic record.",
+         savew);
+  }
+  if (!getparint("mt", &mt)) {
+    mt = 1;
+    warn("<warning>: didn't specify the sampling interval of seismic receiver_p. The program will set it as %d.", mt);
+  }
   if (!getparint("snap_sample", &snap_sample)) {
     snap_sample = 500 / (dt * 1000);
-    warn("<warning>: snap_sample not given, using 500 / (dt * 1000) = %d.", snap_sample);
+    warn("<warning>: didn't specify the sampling interval of snapshot. The program will set it as 500 / (dt * 1000) "
+         "= %d.",
+         snap_sample);
   }
-  if (!getparint("shotID", &shotID)) shotID = 1, warn("<warning>: shotID not given, using %d.", shotID);
-  if (!getparint("energy", &energy)) energy = 0, warn("<warning>: energy not given, using %d.", energy);
+  if (!getparint("shotID", &shotID)) {
+    shotID = 1;
+    warn("<warning>: didn't specify the ID number of shot. The program will set it as %d.", shotID);
+  }
+  if (!getparint("energy", &energy)) {
+    energy = 0;
+    warn("<warning>: didn't specify whether to calculate the energy of wavefiled. The program will set it as %d.",
+         energy);
+  }
   if (!getparint("dumps", &dumps)) dumps = 1; /* parameter/model side-effect files, as the reference writes */
   if (!getparstring("vpfile", &vpfile)) err("<error>: must specify P-wave velocity!");
-  if (!getparstring("vsfile", &vsfile)) warn("<warning>: vsfile not given, vs = 0.577350269189626 * vp (0 where vp <= 1500)!");
-  if (!getparstring("rhofile", &rhofile)) warn("<warning>: rhofile not given, rho = 1741 * (vp/1000)^0.25 (1000 where vp <= 1500)!");
+  if (!getparstring("vsfile", &vsfile))
+    warn("<warning>: didn't specify S-wave velocity! The program will set S-wave velocity as vp / sqrt(3)!");
+  if (!getparstring("rhofile", &rhofile))
+    warn("<warning>: didn't specify density file! The program will set density as 310 * pow(vp, 0.25)!");
   if (!getparstring("epsilonfile", &epsilonfile)) err("<error>: must specify Thomsen parameter (epsilon) file!");
   if (!getparstring("deltafile", &deltafile)) err("<error>: must specify Thomsen parameter (delta) file!");
   if (!getparstring("thetafile", &thetafile)) err("<error>: must specify dip angle (theta) file!");
   if (!getparstring("sourcefile", &sourcefile)) err("<error>: must specify sourcefile!");
-  if (!getparstring("recposfile", &recposfile)) warn("<warning>: recposfile not given!");
-  if (!getparstring("snap_prefix", &snap_prefix)) snap_prefix = "snap", warn("<warning>: snap_prefix not given, using %s.", snap_prefix);
-  if (!getparstring("trace_prefix", &trace_prefix)) trace_prefix = "traces", warn("<warning>: trace_prefix not given, using %s.", trace_prefix);
+  if (!getparstring("recposfile", &recposfile)) warn("<warning>: didn't specify receiver_p position file!");
+  if (!getparstring("snap_prefix", &snap_prefix)) {
+    snap_prefix = "snap";
+    warn("<warning>: didn't specify prefix of snapshot file! The program will set it as %s.", snap_prefix);
+  }
+  if (!getparstring("trace_prefix", &trace_prefix)) {
+    trace_prefix = "traces";
+    warn("<warning>: didn't specify the prefix of traces file. The program will set it as %s", trace_prefix);
+  }
   if (nz < 1 || nx < 1 || nt < 1) err("<error>: nz, nx and nt must be positive!");
   if (mt < 1) err("<error>: mt must be positive!");
   if (wsave > 0 && snap_sample < 1) err("<error>: snap_sample must be positive!");

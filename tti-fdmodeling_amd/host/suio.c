@@ -98,43 +98,73 @@ int head_scalar_read(float key, int scalar)
 
 int head_scalar_write(float key, int scalar) { return head_scalar_read(key, -scalar); }
 
-/* Kaiser window (beta = 6) over the 8-point support |x| < 4 */
-static double kaiser8(double x)
+/* 8-point least-squares optimal interpolation, the method of Seismic Unix's ints8r
+ * (version unpinned; called by read2dsource, src/lib/fdmod/obsinfo.c:82, with zero
+ * outside the trace).  For a fractional offset d in (0, 1) the 8 coefficients w_j
+ * (taps at samples i-3 .. i+4 around x = i + d) minimise the mean square error of
+ * the interpolation over signals band-limited to fmax of the Nyquist frequency, with
+ * fmax = 0.066 + 0.265 ln 8: the normal equations are the symmetric Toeplitz system
+ * sum_k R(j-k) w_k = R(3 - j + d), R(x) = sinc(fmax x).  Offsets are tabulated at
+ * 1/512 of a sample and the nearest entry is used; offsets 0 and 1 take the sample. */
+#define LS8_NTAB 513
+
+static double sincpi(double x) { return x == 0.0 ? 1.0 : sin(M_PI * x) / (M_PI * x); }
+
+/* Levinson recursion for the symmetric Toeplitz system T w = c, T[j][k] = r[|j-k|]. */
+static void toeplitz_solve(int n, const double *r, const double *c, double *w)
 {
-  const double beta = 6.0;
-  double r = x / 4.0;
-  if (fabs(r) >= 1.0) return 0.0;
-  double a = beta * sqrt(1.0 - r * r), s = 1.0, t = 1.0, b = 1.0;
-  /* I0 by its power series, for the argument and for beta */
-  for (int k = 1; k < 30; k++) {
-    t *= (a / (2.0 * k)) * (a / (2.0 * k));
-    s += t;
+  double a[8], tmp[8];
+  double e = r[0];
+  a[0] = 1.0;
+  w[0] = c[0] / r[0];
+  for (int m = 1; m < n; m++) {
+    double k = 0.0;  /* reflection coefficient of the prediction-error filter a */
+    for (int j = 0; j < m; j++) k -= a[j] * r[m - j];
+    k /= e;
+    for (int j = 0; j <= m; j++) tmp[j] = (j < m ? a[j] : 0.0) + k * (m - j < m ? a[m - j] : 0.0);
+    for (int j = 0; j <= m; j++) a[j] = tmp[j];
+    e *= 1.0 - k * k;
+    double q = c[m];  /* extend the solution with the backward filter */
+    for (int j = 0; j < m; j++) q -= w[j] * r[m - j];
+    q /= e;
+    w[m] = 0.0;
+    for (int j = 0; j <= m; j++) w[j] += q * a[m - j];
   }
-  double tb = 1.0;
-  for (int k = 1; k < 30; k++) {
-    tb *= (beta / (2.0 * k)) * (beta / (2.0 * k));
-    b += tb;
+}
+
+static float ls8_table[LS8_NTAB][8];
+static int ls8_ready = 0;
+
+static void ls8_build(void)
+{
+  const double fmax = fmin(0.066 + 0.265 * log(8.0), 1.0);
+  double r[8], c[8], w[8];
+  for (int j = 0; j < 8; j++) r[j] = sincpi(fmax * j);
+  for (int t = 0; t < LS8_NTAB; t++) {
+    for (int j = 0; j < 8; j++) ls8_table[t][j] = 0.0f;
+    if (t == 0) { ls8_table[t][3] = 1.0f; continue; }
+    if (t == LS8_NTAB - 1) { ls8_table[t][4] = 1.0f; continue; }
+    double d = (double)t / (LS8_NTAB - 1);
+    for (int j = 0; j < 8; j++) c[j] = sincpi(fmax * (3.0 - j + d));
+    toeplitz_solve(8, r, c, w);
+    for (int j = 0; j < 8; j++) ls8_table[t][j] = (float)w[j];
   }
-  return s / b;
+  ls8_ready = 1;
 }
 
 void resample8(int nin, float din, float fin, const float *yin, int nout, const float *tout, float *yout)
 {
+  if (!ls8_ready) ls8_build();
   for (int i = 0; i < nout; i++) {
     double xi = ((double)tout[i] - fin) / din;
-    double r = floor(xi + 0.5);
-    if (fabs(xi - r) < 1.0 / 1024.0) {
-      long k = (long)r;
-      yout[i] = (k >= 0 && k < nin) ? yin[k] : 0.0f;
-      continue;
-    }
-    long k0 = (long)floor(xi);
+    double fl = floor(xi);
+    long k0 = (long)fl;
+    int t = (int)((xi - fl) * (LS8_NTAB - 1) + 0.5);
+    const float *w = ls8_table[t];
     double acc = 0.0;
-    for (long k = k0 - 3; k <= k0 + 4; k++) {
-      if (k < 0 || k >= nin) continue;
-      double x = xi - k;
-      double sinc = x == 0.0 ? 1.0 : sin(M_PI * x) / (M_PI * x);
-      acc += yin[k] * sinc * kaiser8(x);
+    for (int j = 0; j < 8; j++) {
+      long k = k0 - 3 + j;
+      if (k >= 0 && k < nin) acc += (double)w[j] * yin[k];
     }
     yout[i] = (float)acc;
   }

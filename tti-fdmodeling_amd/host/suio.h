@@ -37,9 +37,10 @@ int head_scalar_read(float key, int scalar);
 int head_scalar_write(float key, int scalar);
 
 /* Resample a uniformly sampled trace (nin samples, interval din, first sample fin)
- * at the nout times tout[], with zero outside the trace.  Times that fall within
- * 1/1024 of a sample take that sample exactly (the same-interval case of the
- * source files); other times use an 8-point Kaiser-windowed sinc. */
+ * at the nout times tout[], with zero outside the trace: the 8-point least-squares
+ * optimal interpolator of Seismic Unix's ints8r (read2dsource, obsinfo.c:82), with
+ * offsets tabulated at 1/512 of a sample.  Times on a sample (to 1/1024 of one) take
+ * that sample exactly, so equal intervals copy the trace. */
 void resample8(int nin, float din, float fin, const float *yin, int nout, const float *tout, float *yout);
 
 #endif

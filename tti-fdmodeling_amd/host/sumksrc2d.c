@@ -2,9 +2,11 @@
  * sumksrc2d -- write a one-trace SU source file (wavelet + position in cells), the
  * input of the modelling programs' sourcefile.  Same parameters and header fields
  * as the reference utility (src/main/wavelet/sumksrc2d.c:52-160); the wavelets
- * follow src/lib/wavelet/wavelet.c (Ricker :81-99, first derivative of Gaussian :51,
- * derivative of Ricker :112, Gaussian :21): t = it*dt - delay, zero once t > delay,
- * first sample forced to zero for the Ricker family.
+ * follow src/lib/wavelet/wavelet.c (Gaussian :21-37, first derivative of Gaussian
+ * :51-67, Ricker :81-99, derivative of Ricker :113-130): t = it*dt - delay, zero once
+ * t > delay, and the first sample forced to zero for every type (wavelet.c:37, 67, 99,
+ * 130).  The diagnostics for missing or defaulted parameters are the reference's lines
+ * (sumksrc2d.c:61-104).
  *   sumksrc2d nt= dt= sdepth= sx= [wtype=1 A=1 fpeak=10 delay=1.5/fpeak wfile=] > src.su
  */
 #include <math.h>
@@ -41,7 +43,7 @@ static void wavelet(float *w, int wtype, float A, float fpeak, float delay, floa
     }
     w[it] = v;
   }
-  if (wtype == 1 || wtype == 3) w[0] = 0.0f;
+  if (nt > 0) w[0] = 0.0f;
 }
 
 int main(int argc, char **argv)
@@ -52,15 +54,31 @@ int main(int argc, char **argv)
   int nt, wtype, sdepth, sx;
   float dt, A, fpeak, delay;
   char *wfile = NULL;
+  if (!getparstring("wfile", &wfile)) {
+    wfile = NULL;
+    warn("<warning>: didn't specify a wavelet file! The program will use one of wavelet in the document!");
+  }
   if (!getparint("nt", &nt)) err("<error>: must specify number of samples in time dimension!");
-  if (!getparint("wtype", &wtype)) wtype = 1;
-  if (!getparint("sdepth", &sdepth)) err("<error>: must specify source position in 1st dimension!");
-  if (!getparint("sx", &sx)) err("<error>: must specify source position in 2nd dimension!");
-  if (!getparfloat("dt", &dt)) err("<error>: must specify sampling interval in time dimension!");
-  if (!getparfloat("A", &A)) A = 1.0f;
-  if (!getparfloat("fpeak", &fpeak)) fpeak = 10.0f;
-  if (!getparfloat("delay", &delay)) delay = 1.5f / fpeak;
-  getparstring("wfile", &wfile);
+  if (!getparint("wtype", &wtype)) {
+    wtype = 1;
+    warn("<warning>: didn't specify type of wavelet! The program will set it %d!", wtype);
+  }
+  if (!getparint("sdepth", &sdepth)) err("<error>: must specify source position in 2nd dimension!");
+  if (!getparint("sx", &sx)) err("<error>: must specify source position in 1st dimension!");
+  if (!getparfloat("dt", &dt)) err("<error>: must specify sampleing interval in time dimension!");
+  if (!getparfloat("A", &A)) {
+    A = 1.0f;
+    warn("<warning>: didn't specify peak amplitude of wavelet! The program will set it as %10.5f.", A);
+  }
+  if (!getparfloat("fpeak", &fpeak)) {
+    fpeak = 10.0f;
+    warn("<warning>: didn't specify fpeak frequency of wavelet! The program will set it as %10.5f Hz.", fpeak);
+  }
+  if (!getparfloat("delay", &delay)) {
+    delay = 1.5f / fpeak;
+    warn("<warning>: didn't specify delay time of wavelet! The program will set delay = 1.5 / fpeak =%10.5f Hz!",
+         delay);
+  }
   if (nt < 1 || nt > 65535) err("<error>: nt must be in 1..65535 (SU ns field)!");
   su_trace tr;
   su_init(&tr);
