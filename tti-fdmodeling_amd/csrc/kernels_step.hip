@@ -86,6 +86,8 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
   const bool und = segment_undamped(g, a.undamped, btz, xa, nc);
   const int iz = z0 + lane;
   const bool zok = iz < g.izEnd;
+  const int izc = min(iz, g.Nz - 1);  // an in-grid row for the look-ahead loads
+  const int xe = xa + nc - 1;
   const int rs = z0 - 4;  // first staged row
   const float *const fs[3] = {a.sxx, a.szz, a.sxz};
   // prologue: columns xa-4 .. xa+4 into slots 0..8; the next PD columns and operands in flight
@@ -112,8 +114,10 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
       const int c = xa + k;
       Col<3> cn;
       VOps on;
-      if (k <= nc - PD - 1) load_col<3>(cn, fs, c + 5 + PD, rs, lane, g);
-      if (zok && k + PD < nc) load_vops<COORD>(on, a, (size_t)(c + PD) * g.ldz + iz, und, g.dt);
+      // look-ahead loads with no branch around them (clamped column and row; the values
+      // past the segment or the grid are not used)
+      load_col_nb<3>(cn, fs, min(c + 5 + PD, xe + 5), rs, lane, g);
+      load_vops<COORD>(on, a, (size_t)min(c + PD, xe) * g.ldz + izc, und, g.dt);
       store_col<3>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
       float hv[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // coordinate-form history (ESTORE)
@@ -206,9 +210,36 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
   const bool fs_top = a.freesurface && btz == 0;
   const int iz = z0 + lane;
   const bool zok = iz < g.izEnd;
+  const int izc = min(iz, g.Nz - 1);  // an in-grid row for the look-ahead loads
   const int rs = z0 - 5;
   const float *const fv[2] = {a.vx, a.vz};
   const int xe = xa + nc - 1;
+  // The fused source's sample and tap weights, and the segment's fused receivers, are
+  // loaded here, once: a load inside the column loop that is used in the same iteration
+  // would make the compiler wait for every load in flight, look-ahead included.
+  __shared__ float tapca[(2 * kN + 1) * (2 * kN + 1)];
+  const bool src_here = (a.fuse & 1) && a.fsx + kN >= xa && a.fsx - kN <= xe;
+  float fsv = 0.0f;
+  if (src_here) {
+    fsv = a.fsrc[a.fit];
+    for (int t = lane; t < (2 * kN + 1) * (2 * kN + 1); t += TZ) tapca[t] = a.ftap[t].ca;
+  }
+  // receivers: for segments of fewer than 64 columns lane l holds rc_start[xa + l]
+  // (l <= nc) and, when the segment has at most 64 receivers, the row and id of its l-th
+  // receiver (CSR order); longer segments read the CSR in the loop
+  const bool rec_strip = (a.fuse & 2) && z0 <= a.rc_zhi && z0 + TZ > a.rc_zlo;
+  int rcs = 0, rcz = -1, rid = 0, j00 = 0;
+  bool rpre = false;
+  if (rec_strip && nc < TZ) {
+    rcs = lane <= nc ? a.rc_start[xa + lane] : 0;
+    j00 = __shfl(rcs, 0, 64);
+    const int cnt = __shfl(rcs, nc, 64) - j00;
+    rpre = cnt <= TZ;
+    if (rpre && lane < cnt) {
+      rcz = a.rc_z[j00 + lane];
+      rid = a.rc_id[j00 + lane];
+    }
+  }
   {
     Col<2> c[9];
 #pragma unroll
@@ -232,8 +263,10 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
       const int c = REV ? xe - k : xa + k;
       Col<2> cn;
       SOps on;
-      if (k <= nc - PD - 1) load_col<2>(cn, fv, REV ? c - 5 - PD : c + 4 + PD, rs, lane, g);
-      if (zok && k + PD < nc) load_sops<FORM>(on, a, (size_t)(REV ? c - PD : c + PD) * g.ldz + iz, und, g.dt);
+      // look-ahead loads with no branch around them (clamped column and row; the values
+      // past the segment or the grid are not used)
+      load_col_nb<2>(cn, fv, REV ? max(c - 5 - PD, xa - 5) : min(c + 4 + PD, xe + 4), rs, lane, g);
+      load_sops<FORM>(on, a, (size_t)(REV ? max(c - PD, xa) : min(c + PD, xe)) * g.ldz + izc, und, g.dt);
       store_col<2>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
       auto slot = [](int dx) { return REV ? (u - dx + 4 + kRing) % kRing : (u + dx + 5 + kRing) % kRing; };
@@ -294,11 +327,22 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
           nzz = S2 * t1 + K2 * t2 - S2T * t3;
           nxz = -0.5f * S2T * (t1 - t2) + (K2 - S2) * t3;
         }
-      } else if (fs_top && iz < g.Nz) {
-        // rows past izEnd are not updated; their current values feed the images
-        nxx = a.sxx[o];
-        nzz = a.szz[o];
-        nxz = a.sxz[o];
+      } else if (FORM != FDMOD_FORM_TENSOR && fs_top && iz < g.Nz) {
+        // rows past izEnd are not updated; their current values feed the images.  The
+        // Bond and coordinate forms have them from the look-ahead loads (a load here
+        // would make the compiler wait for every load in flight, in every strip).
+        nxx = opq[0].s1;
+        nzz = opq[0].s2;
+        nxz = opq[0].s3;
+      }
+      // the tensor form's look-ahead holds tau, not sigma: a top strip that reaches past
+      // izEnd (grids less than 64 rows deep) loads sigma there, under a uniform branch
+      if (FORM == FDMOD_FORM_TENSOR && fs_top && z0 + TZ > g.izEnd) {
+        if (!zok && iz < g.Nz) {
+          nxx = a.sxx[o];
+          nzz = a.szz[o];
+          nxz = a.sxz[o];
+        }
       }
       // free-surface images of row izStart+lane into row izStart-lane (lanes 1..kN)
       const bool img = fs_top && lane >= 1 && lane <= kN;
@@ -315,19 +359,18 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
           ixz = -nxz;
         }
       }
-      if (a.fuse & 1) {  // stress source (BT:893-914) after the images, tap by tap as k_source adds it
+      if (src_here) {  // stress source (BT:893-914) after the images, tap by tap as k_source adds it
         const int ox = c - a.fsx;
         if (ox >= -kN && ox <= kN) {
-          const float sv = a.fsrc[a.fit];
-          const SrcTap *tp = a.ftap + (ox + kN) * (2 * kN + 1);
+          const float *tp = tapca + (ox + kN) * (2 * kN + 1);
           const int oz = iz - a.fsz, om = z0 - lane - a.fsz;
           if (zok && oz >= -kN && oz <= kN) {
-            const float da = tp[oz + kN].ca * sv;
+            const float da = tp[oz + kN] * fsv;
             nxx = nxx + da;
             nzz = nzz + da;
           }
           if (img && om >= -kN && om <= kN) {
-            const float da = tp[om + kN].ca * sv;
+            const float da = tp[om + kN] * fsv;
             ixx = ixx + da;
             izz = izz + da;
           }
@@ -345,9 +388,20 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
         st_out(a.sxz + o, nxz);
       }
       // pressure receivers of this column (isoelastic.c:103-144), in strips that hold some
-      if ((a.fuse & 2) && z0 <= a.rc_zhi && z0 + TZ > a.rc_zlo)
-        for (int j = a.rc_start[c]; j < a.rc_start[c + 1]; j++)
-          if (zok && a.rc_z[j] == iz) a.rec_p[a.rec_off + a.rc_id[j]] = 0.5f * (nxx + nzz);
+      if (rec_strip) {
+        const int kc = c - xa;
+        const int j0 = nc < TZ ? __shfl(rcs, kc, 64) : a.rc_start[c];
+        const int j1 = nc < TZ ? __shfl(rcs, kc + 1, 64) : a.rc_start[c + 1];
+        if (rpre) {  // the segment's receivers are in the lanes
+          for (int j = j0; j < j1; j++) {
+            const int z = __shfl(rcz, j - j00, 64), id = __shfl(rid, j - j00, 64);
+            if (zok && z == iz) a.rec_p[a.rec_off + id] = 0.5f * (nxx + nzz);
+          }
+        } else {
+          for (int j = j0; j < j1; j++)
+            if (zok && a.rc_z[j] == iz) a.rec_p[a.rec_off + a.rc_id[j]] = 0.5f * (nxx + nzz);
+        }
+      }
       if (ESTORE) {
         if (FORM == FDMOD_FORM_BOND) {
           const float e[3] = {hv[0], hv[1], hv[2]};
@@ -376,16 +430,15 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
 #define FDMOD_PIPE_V 2
 #endif
 #ifndef FDMOD_PIPE_S
-#define FDMOD_PIPE_S 2
+#define FDMOD_PIPE_S 1
 #endif
 constexpr int kPipeV = FDMOD_PIPE_V;  // prefetch depth (iterations) of the velocity sweep
-// Stress sweep: one column ahead for the Bond and coordinate forms, whose operands then
-// fit in fewer registers (71 VGPRs against 91; with a history store 79-85 against
-// 101-107, five or six waves per SIMD instead of four), two for the tensor form (more
-// operands per point).  Measured on the 1256 x 8002 grid: Bond +0.4%, coordinate +0.6%,
-// tensor -0.5% at depth 1; FP16-history gradients 0.5% faster.
+// Stress sweep: one column ahead for every form (Bond and coordinate 89 VGPRs, tensor 91,
+// five waves per SIMD; the tensor form at depth 2 needs 103, four waves).  Measured on
+// the 1256 x 8002 grid with branch-free look-ahead loads: tensor stress sweep 134.7 us at
+// depth 1 against 140.8 us at depth 2.
 template <int FORM>
-constexpr int pipe_s() { return FORM == FDMOD_FORM_TENSOR ? FDMOD_PIPE_S : 1; }
+constexpr int pipe_s() { return FDMOD_PIPE_S; }
 
 // Single grid: blocks map to tiles of one shot.
 template <bool COORD, int PD, int ESTORE = 0>
@@ -477,26 +530,38 @@ static dim3 sweep_grid(const Grid &g, int segL)
 
 int sweep_tiles(const Grid &g, int segL) { return g.tilesZ * ((g.ixEnd - g.ixStart + segL - 1) / segL); }
 
+template <bool COORD, int ESTORE>
+static void launch_v(hipStream_t s, dim3 grid, const SweepArgs &a)
+{
+  hipLaunchKernelGGL((k_vsweep<COORD, kPipeV, ESTORE>), grid, dim3(64), 0, s, a);
+}
+
+template <int FORM, int ESTORE>
+static void launch_s(hipStream_t s, dim3 grid, const SweepArgs &a)
+{
+  hipLaunchKernelGGL((k_ssweep<FORM, pipe_s<FORM>(), ESTORE>), grid, dim3(64), 0, s, a);
+}
+
 void launch_vsweep(hipStream_t s, int form, const SweepArgs &a)
 {
   dim3 grid = sweep_grid(a.g, a.g.segLV);
   if (form == FDMOD_FORM_COORD) {
-    if (a.estore == 1) hipLaunchKernelGGL((k_vsweep<true, kPipeV, 1>), grid, dim3(64), 0, s, a);
-    else if (a.estore == 2) hipLaunchKernelGGL((k_vsweep<true, kPipeV, 2>), grid, dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((k_vsweep<true, kPipeV>), grid, dim3(64), 0, s, a);
+    if (a.estore == 1) launch_v<true, 1>(s, grid, a);
+    else if (a.estore == 2) launch_v<true, 2>(s, grid, a);
+    else launch_v<true, 0>(s, grid, a);
   } else {
-    hipLaunchKernelGGL((k_vsweep<false, kPipeV>), grid, dim3(64), 0, s, a);
+    launch_v<false, 0>(s, grid, a);
   }
 }
 
 void launch_ssweep(hipStream_t s, int form, const SweepArgs &a)
 {
   dim3 grid = sweep_grid(a.g, a.g.segLS);
-#define FDMOD_SSWEEP(F)                                                                            \
-  do {                                                                                           \
-    if (a.estore == 1) hipLaunchKernelGGL((k_ssweep<F, pipe_s<F>(), 1>), grid, dim3(64), 0, s, a);    \
-    else if (a.estore == 2) hipLaunchKernelGGL((k_ssweep<F, pipe_s<F>(), 2>), grid, dim3(64), 0, s, a); \
-    else hipLaunchKernelGGL((k_ssweep<F, pipe_s<F>()>), grid, dim3(64), 0, s, a);                     \
+#define FDMOD_SSWEEP(F)                                      \
+  do {                                                     \
+    if (a.estore == 1) launch_s<F, 1>(s, grid, a);          \
+    else if (a.estore == 2) launch_s<F, 2>(s, grid, a);     \
+    else launch_s<F, 0>(s, grid, a);                        \
   } while (0)
   if (form == FDMOD_FORM_COORD) FDMOD_SSWEEP(FDMOD_FORM_COORD);
   else if (form == FDMOD_FORM_TENSOR) FDMOD_SSWEEP(FDMOD_FORM_TENSOR);

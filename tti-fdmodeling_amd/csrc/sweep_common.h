@@ -101,6 +101,29 @@ __device__ __forceinline__ void load_col(Col<NF> &c, const float *const (&f)[NF]
   }
 }
 
+// The same column load with no branch around the loads: rows past the grid read row
+// Nz-1 and yield zero, and lanes >= kHalo re-read their own row for the unused hi value.
+// The sweeps issue their look-ahead loads this way (from clamped columns too): a load
+// under a branch makes the compiler's wait before the next use of an older load count
+// the path that skipped it, which waits for every load in flight -- the look-ahead is
+// then lost.
+template <int NF>
+__device__ __forceinline__ void load_col_nb(Col<NF> &c, const float *const (&f)[NF], int gx, int rs, int lane,
+                                            const Grid &g)
+{
+  const size_t base = (size_t)gx * g.ldz;
+  const int r0 = rs + lane, r1 = rs + TZ + lane;
+  const bool ok0 = r0 < g.Nz, ok1 = lane < kHalo && r1 < g.Nz;
+  const int a0 = ok0 ? r0 : g.Nz - 1, a1 = ok1 ? r1 : a0;
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    const float *col = f[k] + base;
+    const float v0 = ldst(col + a0), v1 = ldst(col + a1);
+    c.lo[k] = ok0 ? v0 : 0.0f;
+    c.hi[k] = ok1 ? v1 : 0.0f;
+  }
+}
+
 template <int NF>
 __device__ __forceinline__ void store_col(float (*ring)[kRing][LZ], int slot, const Col<NF> &c, int lane)
 {
