@@ -33,7 +33,9 @@ constexpr int kRedBlocks = 256;  // partial sums of the FP64 reductions
 constexpr int kMaxState = 8;     // vx, vz, sxx, szz, sxz (+ txx, tzz, txz for the tensor form)
 constexpr int kNG = 8;           // imaging sums
 constexpr int kAdjSegS = 44, kAdjSegV = 44;  // longest adjoint segments (columns)
-constexpr int kAdjMinTiles = 1450;            // blocks per adjoint sweep below which they shorten
+// blocks per adjoint sweep below which they shorten: the stress adjoint counts strips of
+// kTZS = 55 output rows, the velocity adjoint strips of 64
+constexpr int kAdjMinTilesS = 1700, kAdjMinTilesV = 1450;
 
 size_t field_floats(const fdmod_ctx *c) { return (size_t)c->g.Nx * c->g.ldz; }
 
@@ -181,18 +183,19 @@ void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
 // segments (fewer, longer blocks) down to a smaller block count.  On 1126-deep crops
 // 1500, 2000 and 3000 columns wide the fastest pairs were 22/22, 22/22 and 33/33
 // columns, 3-7% faster backward passes than 11-column segments; the longest segment
-// giving at least kAdjMinTiles blocks picks each of them.
+// giving at least kAdjMinTilesS / kAdjMinTilesV blocks picks each of them.
 void adj_segments(fdmod_ctx *c)
 {
   if (c->adj_segS > 0) return;
   const Grid &g = c->g;
-  auto pick = [&](int longest, int min_tiles) {
+  // strips: the stress adjoint outputs kTZS rows per strip, the velocity adjoint TZ
+  auto pick = [&](int longest, int min_tiles, int strips) {
     int L = longest;
-    while (L > kFlagW && g.tilesZ * ((g.ixEnd - g.ixStart + L - 1) / L) < min_tiles) L -= kFlagW;
+    while (L > kFlagW && strips * ((g.ixEnd - g.ixStart + L - 1) / L) < min_tiles) L -= kFlagW;
     return L;
   };
-  c->adj_segS = pick(kAdjSegS, kAdjMinTiles);
-  c->adj_segV = pick(kAdjSegV, kAdjMinTiles);
+  c->adj_segS = pick(kAdjSegS, kAdjMinTilesS, (g.izEnd - g.izStart + kTZS - 1) / kTZS);
+  c->adj_segV = pick(kAdjSegV, kAdjMinTilesV, g.tilesZ);
   if (const char *e = getenv("FDMOD_ADJ_SEGS")) {  // tile-shape overrides (tuning experiments)
     const int v = atoi(e);
     if (v >= 1 && v <= 4096) c->adj_segS = v;
@@ -211,6 +214,8 @@ AdjArgs adj_args(fdmod_ctx *c, bool image_rows)
   a.g.segLV = c->adj_segS;  // the stress adjoint is tiled like the velocity sweep
   a.g.segLS = c->adj_segV;  // and the velocity adjoint like the stress sweep
   a.izU = c->g.izStart;
+  a.undamped = c->flags;
+  a.ftz = c->g.tilesZ;
   if (image_rows && c->p.freesurface) {
     a.g.izStart -= kN;
     a.g.tilesZ = (a.g.izEnd - a.g.izStart + TZ - 1) / TZ;

@@ -146,6 +146,8 @@ struct AdjArgs {
   const float *coef1, *coef2, *invrho;
   const float *c11, *c13, *c15, *c33, *c35, *c55;
   const float *sin2, *cos2, *sin2t, *sn, *cs, *snv, *csv;
+  const uint8_t *undamped;             // the forward's per-tile flags (coef1 == 1, coef2 == dt), tiles
+  int ftz;                             // of 64 rows from row izU and kFlagW columns; ftz strips
   float *oxx, *ozz, *oxz;              // V-adjoint: where the updated mu_sigma goes (mxx... in place)
   float *snap[6];                      // V-adjoint: imaging snapshots of computed fields, [3..] (or null)
   double *G[8];                        // imaging sums (per-form meaning, kernels_adjoint.hip)

@@ -33,24 +33,44 @@ namespace fdm {
 //           transpose of the strain rotation, for (vxdx, vzdz, vxdz + vzdx)    3 fields
 //   coord   VTI adjoint of the rotated gradients, rotated back, for
 //           (vxdx, vxdz, vzdx, vzdz)                                            4 fields
+// True when every point of the update range U inside rows [z_lo, z_hi) and columns
+// [x_lo, x_hi) is undamped (coef1 == 1.0f, coef2 == dt): a block whose staged and output
+// points all qualify uses those constants and loads neither coefficient.  Points outside
+// U need no check, the adjoint fields the coefficients multiply there being zero.
+__device__ __forceinline__ bool region_undamped(const AdjArgs &a, int z_lo, int z_hi, int x_lo, int x_hi)
+{
+  const Grid &g = a.g;
+  z_lo = max(z_lo, a.izU);
+  z_hi = min(z_hi, g.izEnd);
+  x_lo = max(x_lo, g.ixStart);
+  x_hi = min(x_hi, g.ixEnd);
+  if (z_lo >= z_hi || x_lo >= x_hi) return true;
+  bool und = true;
+  for (int tz = (z_lo - a.izU) / TZ; tz <= (z_hi - 1 - a.izU) / TZ; tz++)
+    for (int f = (x_lo - g.ixStart) / kFlagW; f <= (x_hi - 1 - g.ixStart) / kFlagW; f++)
+      und = und && a.undamped[f * a.ftz + tz] != 0;
+  return und;
+}
+
 constexpr int kRowMax = 14;
 struct MRow {
   float v[kRowMax];
 };
 
 template <int FORM>
-__device__ __forceinline__ void load_mrow(MRow &r, const AdjArgs &a, size_t o)
+__device__ __forceinline__ void load_mrow(MRow &r, const AdjArgs &a, size_t o, bool und)
 {
+  const float k2 = und ? a.g.dt : a.coef2[o];
   r.v[0] = a.mxx[o]; r.v[1] = a.mzz[o]; r.v[2] = a.mxz[o];
   r.v[3] = a.c11[o]; r.v[4] = a.c13[o];
   if (FORM == FDMOD_FORM_BOND) {
-    r.v[5] = a.c15[o]; r.v[6] = a.c33[o]; r.v[7] = a.c35[o]; r.v[8] = a.c55[o]; r.v[9] = a.coef2[o];
+    r.v[5] = a.c15[o]; r.v[6] = a.c33[o]; r.v[7] = a.c35[o]; r.v[8] = a.c55[o]; r.v[9] = k2;
   } else if (FORM == FDMOD_FORM_TENSOR) {
-    r.v[5] = a.c33[o]; r.v[6] = a.c55[o]; r.v[7] = a.coef2[o];
+    r.v[5] = a.c33[o]; r.v[6] = a.c55[o]; r.v[7] = k2;
     r.v[8] = a.mtxx[o]; r.v[9] = a.mtzz[o]; r.v[10] = a.mtxz[o];
     r.v[11] = a.sin2[o]; r.v[12] = a.cos2[o]; r.v[13] = a.sin2t[o];
   } else {
-    r.v[5] = a.c33[o]; r.v[6] = a.c55[o]; r.v[7] = a.coef2[o]; r.v[8] = a.sn[o]; r.v[9] = a.cs[o];
+    r.v[5] = a.c33[o]; r.v[6] = a.c55[o]; r.v[7] = k2; r.v[8] = a.sn[o]; r.v[9] = a.cs[o];
   }
 }
 
@@ -104,10 +124,10 @@ struct VAdjOps {
 // Staged column of the stress adjoint: lane l holds row rs + l only (the strip's 64
 // staged rows are its kTZS = 55 output rows and their 4 + 5 halo rows).
 template <int FORM>
-__device__ __forceinline__ void load_mrow_lane(MRow &r, const AdjArgs &a, int gx, int row, const Grid &g)
+__device__ __forceinline__ void load_mrow_lane(MRow &r, const AdjArgs &a, int gx, int row, const Grid &g, bool und)
 {
   r = MRow{};
-  if (row >= 0 && row < g.Nz) load_mrow<FORM>(r, a, (size_t)gx * g.ldz + row);
+  if (row >= 0 && row < g.Nz) load_mrow<FORM>(r, a, (size_t)gx * g.ldz + row, und);
 }
 
 template <int FORM>
@@ -137,12 +157,14 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   const int rs = z0 - 4;
   const int iz = rs + lane;
   const bool zok = lane >= 4 && lane < 4 + kTZS && iz < g.izEnd;
+  // staged rows rs .. rs+63 of columns xa-4 .. xa+nc+4
+  const bool und = region_undamped(a, rs, rs + TZ, xa - 4, xa + nc + 5);
   // prologue in groups of three columns (a staged row is up to 14 floats)
 #pragma unroll
   for (int m0 = 0; m0 < 9; m0 += 3) {
     MRow c[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) load_mrow_lane<FORM>(c[m], a, xa - 4 + m0 + m, iz, g);
+    for (int m = 0; m < 3; m++) load_mrow_lane<FORM>(c[m], a, xa - 4 + m0 + m, iz, g, und);
 #pragma unroll
     for (int m = 0; m < 3; m++) store_mrow<FORM>(ring, m0 + m, c[m], lane);
   }
@@ -152,11 +174,11 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
     const size_t o = (size_t)col * g.ldz + iz;
     p.mvx = a.mvx[o];
     p.mvz = a.mvz[o];
-    p.k1 = a.coef1[o];
+    p.k1 = und ? 1.0f : a.coef1[o];
   };
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    if (i <= nc - 1) load_mrow_lane<FORM>(colq[i], a, xa + 5 + i, iz, g);
+    if (i <= nc - 1) load_mrow_lane<FORM>(colq[i], a, xa + 5 + i, iz, g, und);
     if (zok && i < nc) load_ops(opq[i], xa + i);
   }
   const float invdx = g.invdx, invdz = g.invdz;
@@ -168,7 +190,7 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
       const int c = xa + k;
       MRow cn;
       VAdjOps on;
-      if (k <= nc - PD - 1) load_mrow_lane<FORM>(cn, a, c + 5 + PD, iz, g);
+      if (k <= nc - PD - 1) load_mrow_lane<FORM>(cn, a, c + 5 + PD, iz, g, und);
       if (zok && k + PD < nc) load_ops(on, c + PD);
       store_mrow<FORM>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
@@ -219,9 +241,9 @@ struct NCol {
 };
 
 template <int FORM>
-__device__ __forceinline__ void load_nrow(NRow &r, const AdjArgs &a, size_t o)
+__device__ __forceinline__ void load_nrow(NRow &r, const AdjArgs &a, size_t o, bool und)
 {
-  r = NRow{a.mvx[o], a.mvz[o], a.coef2[o], a.invrho[o], 0.0f, 0.0f};
+  r = NRow{a.mvx[o], a.mvz[o], und ? a.g.dt : a.coef2[o], a.invrho[o], 0.0f, 0.0f};
   if (FORM == FDMOD_FORM_COORD) {
     r.cv = a.csv[o];
     r.sv = a.snv[o];
@@ -229,14 +251,14 @@ __device__ __forceinline__ void load_nrow(NRow &r, const AdjArgs &a, size_t o)
 }
 
 template <int FORM>
-__device__ __forceinline__ void load_ncol(NCol &c, const AdjArgs &a, int gx, int rs, int lane, const Grid &g)
+__device__ __forceinline__ void load_ncol(NCol &c, const AdjArgs &a, int gx, int rs, int lane, const Grid &g, bool und)
 {
   const size_t base = (size_t)gx * g.ldz;
   const int r0 = rs + lane, r1 = rs + TZ + lane;
   c.lo = NRow{0, 0, 0, 0, 0, 0};
   c.hi = NRow{0, 0, 0, 0, 0, 0};
-  if (r0 >= 0 && r0 < g.Nz) load_nrow<FORM>(c.lo, a, base + r0);
-  if (lane < kHalo && r1 >= 0 && r1 < g.Nz) load_nrow<FORM>(c.hi, a, base + r1);
+  if (r0 >= 0 && r0 < g.Nz) load_nrow<FORM>(c.lo, a, base + r0, und);
+  if (lane < kHalo && r1 >= 0 && r1 < g.Nz) load_nrow<FORM>(c.hi, a, base + r1, und);
 }
 
 template <int FORM>
@@ -303,11 +325,13 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
   const bool zok = iz < g.izEnd;
   const bool inU = iz >= a.izU;
   const int rs = z0 - 5;
+  // staged rows rs .. rs+72 of columns xa-5 .. xa+nc+3
+  const bool und = region_undamped(a, rs, rs + LZ, xa - 5, xa + nc + 4);
 #pragma unroll
   for (int m0 = 0; m0 < 9; m0 += 3) {
     NCol c[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) load_ncol<FORM>(c[m], a, xa - 5 + m0 + m, rs, lane, g);
+    for (int m = 0; m < 3; m++) load_ncol<FORM>(c[m], a, xa - 5 + m0 + m, rs, lane, g, und);
 #pragma unroll
     for (int m = 0; m < 3; m++) store_ncol<FORM>(ring, m0 + m, c[m], lane);
   }
@@ -319,7 +343,7 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
     p.s1 = a.mxx[o];
     p.s2 = a.mzz[o];
     p.s3 = a.mxz[o];
-    p.k1 = inU ? a.coef1[o] : 1.0f;
+    p.k1 = inU && !und ? a.coef1[o] : 1.0f;
     if (FORM == FDMOD_FORM_TENSOR) {
       p.t1 = a.mtxx[o];
       p.t2 = a.mtzz[o];
@@ -329,14 +353,14 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
       p.S2T = inU ? a.sin2t[o] : 0.0f;
     }
     if (FORM == FDMOD_FORM_COORD && a.snap[3] && inU) {
-      const float kr = a.coef2[o] * a.invrho[o];
+      const float kr = (und ? g.dt : a.coef2[o]) * a.invrho[o];
       p.na = kr * a.mvx[o];
       p.nb = kr * a.mvz[o];
     }
   };
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    if (i <= nc - 1) load_ncol<FORM>(colq[i], a, xa + 4 + i, rs, lane, g);
+    if (i <= nc - 1) load_ncol<FORM>(colq[i], a, xa + 4 + i, rs, lane, g, und);
     if (zok && i < nc) load_ops(opq[i], xa + i);
   }
   const float invdx = g.invdx, invdz = g.invdz;
@@ -348,7 +372,7 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
       const int c = xa + k;
       NCol cn;
       SAdjOps on;
-      if (k <= nc - PD - 1) load_ncol<FORM>(cn, a, c + 4 + PD, rs, lane, g);
+      if (k <= nc - PD - 1) load_ncol<FORM>(cn, a, c + 4 + PD, rs, lane, g, und);
       if (zok && k + PD < nc) load_ops(on, c + PD);
       store_ncol<FORM>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
