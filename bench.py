@@ -222,9 +222,9 @@ def batched_vs_loop(form, m, cfg, steps, nshots=64, seed=3):
 GRAD_CFG = dict(nz=1126, dx=10.0, dz=10.0, dt=8e-4, nt=4000, ealThick=60, freesurface=0, fpeak=12.0, sz=2, rz=2)
 
 
-def _grad_shot(cls, m, w, sx, nt, **kw):
+def _grad_shot(cls, m, w, sx, nt, form="bond", **kw):
     c = GRAD_CFG
-    g = cls("bond", c["nz"], w, nt, c["dx"], c["dz"], c["dt"], ealThick=c["ealThick"], freesurface=c["freesurface"],
+    g = cls(form, c["nz"], w, nt, c["dx"], c["dz"], c["dt"], ealThick=c["ealThick"], freesurface=c["freesurface"],
             fpeak=c["fpeak"], **kw)
     g.set_model(m["vp"], m["vs"], m["rho"], m["eps"], m["delta"], m["theta"])
     from fdmod_hip import synth
@@ -238,41 +238,56 @@ def _rel(a, b):
     return [float(np.linalg.norm(a[i] - b[i]) / max(np.linalg.norm(b[i]), 1e-300)) for i in range(4)]
 
 
-def gradient_single(full, nt):
-    """Adjoint gradient workload: one shot on a 1126 x 2000 crop, nt steps; the plain
-    FP32 checkpointed gradient, the FP16 full-history gradient and the FP32
-    boundary-saving reconstruction (exact, and without the absorbing layers' imaging)."""
+GRAD_MODES = {"plain_f32_checkpoints": dict(), "f16_history": dict(storage="f16", history=True),
+              "f32_reconstruct": dict(history="reconstruct"),
+              "f32_reconstruct_interior": dict(history="reconstruct_interior")}
+
+
+def gradient_single(full, nt, modes=tuple(GRAD_MODES), forms=("bond", "tensor", "coord")):
+    """Adjoint gradient workload: one shot on a 1126 x 2000 crop, nt steps.  Bond form:
+    the plain FP32 checkpointed gradient, the FP16 full-history gradient and the FP32
+    boundary-saving reconstruction (exact, and without the absorbing layers' imaging);
+    tensor and coordinate forms: the plain and the FP16-history gradients."""
     from fdmod_hip import Modeling, synth
 
     W, x0 = 2000, 2936
     bg = synth.crop(full, x0, W)
     tr = synth.add_anomalies(bg)  # five anomalies inside the crop
-    gt = _grad_shot(Modeling, tr, W, W // 2, nt)
-    gt.run()
-    obs = gt.traces(4).copy()
-    gt.close()
-    g = _grad_shot(Modeling, bg, W, W // 2, nt)
-    out = {"workload": f"one shot, Bond form, 1126 x {W} crop of the seeded model (columns {x0}..{x0 + W - 1}), "
-                       f"true model = background + 5 Gaussian anomalies inside the crop at 0.8-2.8 km depth, nt={nt}, dt=0.8 ms, no free surface, "
-                       f"{W} pressure receivers; gradient wrt vp, eps, delta, theta"}
-    res = {}
-    for name, kw in (("plain_f32_checkpoints", dict()), ("f16_history", dict(storage="f16", history=True)),
-                     ("f32_reconstruct", dict(history="reconstruct")),
-                     ("f32_reconstruct_interior", dict(history="reconstruct_interior"))):
-        g.gradient(obs_p=obs, **kw)  # warm-up (allocations)
-        gpu_sync()
-        t0 = time.perf_counter()
-        J, gr = g.gradient(obs_p=obs, **kw)
-        wall = time.perf_counter() - t0
-        res[name] = gr
-        out[name] = {"seconds": round(wall, 4), "gradients_per_hour": round(3600.0 / wall, 1), "misfit": J}
-    out["f16_vs_f32_rel_l2"] = [round(x, 7) for x in _rel(res["f16_history"], res["plain_f32_checkpoints"])]
-    out["reconstruct_vs_f32_rel_l2"] = [float("%.3g" % x)
-                                        for x in _rel(res["f32_reconstruct"], res["plain_f32_checkpoints"])]
-    inner = (slice(None), slice(1, -1), slice(1, -1))  # off the model's edge cells
-    out["reconstruct_interior_vs_f32_rel_l2_inner"] = [
-        float("%.3g" % x) for x in _rel(res["f32_reconstruct_interior"][inner], res["plain_f32_checkpoints"][inner])]
-    g.close()
+    out = {"workload": f"one shot, 1126 x {W} crop of the seeded model (columns {x0}..{x0 + W - 1}), true model "
+                       f"= background + 5 Gaussian anomalies inside the crop at 0.8-2.8 km depth, nt={nt}, dt=0.8 ms, "
+                       f"no free surface, {W} pressure receivers; gradient wrt vp, eps, delta, theta"}
+    for form in forms:
+        gt = _grad_shot(Modeling, tr, W, W // 2, nt, form=form)
+        gt.run()
+        obs = gt.traces(4).copy()
+        gt.close()
+        g = _grad_shot(Modeling, bg, W, W // 2, nt, form=form)
+        res, fo = {}, {}
+        for name in modes:
+            if form != "bond" and name.startswith("f32_reconstruct"):
+                continue
+            kw = GRAD_MODES[name]
+            g.gradient(obs_p=obs, **kw)  # warm-up (allocations)
+            gpu_sync()
+            t0 = time.perf_counter()
+            J, gr = g.gradient(obs_p=obs, **kw)
+            wall = time.perf_counter() - t0
+            res[name] = gr
+            fo[name] = {"seconds": round(wall, 4), "gradients_per_hour": round(3600.0 / wall, 1), "misfit": J}
+        ref = res.get("plain_f32_checkpoints")
+        if ref is not None and "f16_history" in res:
+            fo["f16_vs_f32_rel_l2"] = [round(x, 7) for x in _rel(res["f16_history"], ref)]
+        if ref is not None and "f32_reconstruct" in res:
+            fo["reconstruct_vs_f32_rel_l2"] = [float("%.3g" % x) for x in _rel(res["f32_reconstruct"], ref)]
+        if ref is not None and "f32_reconstruct_interior" in res:
+            inner = (slice(None), slice(1, -1), slice(1, -1))  # off the model's edge cells
+            fo["reconstruct_interior_vs_f32_rel_l2_inner"] = [
+                float("%.3g" % x) for x in _rel(res["f32_reconstruct_interior"][inner], ref[inner])]
+        g.close()
+        if form == "bond":
+            out.update(fo)  # the Bond form's figures stay at the top level
+        else:
+            out[form] = fo
     return out
 
 
@@ -425,17 +440,23 @@ def main():
     ap.add_argument("--grad-steps", type=int, default=4000)
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--grad-only", action="store_true", help="only the single-shot gradient workload (profiling)")
+    ap.add_argument("--grad-modes", default=",".join(GRAD_MODES), help="single-shot gradient modes (comma list)")
+    ap.add_argument("--grad-forms", default="bond,tensor,coord", help="forms of the single-shot gradient workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU shot sharding is a design note only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    modes = tuple(x for x in args.grad_modes.split(",") if x)
+    gforms = tuple(x for x in args.grad_forms.split(",") if x)
+    if any(x not in GRAD_MODES for x in modes):
+        ap.error(f"--grad-modes: choose from {','.join(GRAD_MODES)}")
     if args.grad_only:
         from fdmod_hip import synth
 
         log("bench: gradient workload only")
         full = synth.model_large()
-        print(json.dumps({"gradients": {"single_shot": gradient_single(full, args.grad_steps)}}))
+        print(json.dumps({"gradients": {"single_shot": gradient_single(full, args.grad_steps, modes, gforms)}}))
         return
     nt = max(args.steps, args.cpu_steps, 1)
     log(f"bench: building the 1126x7872 workload (nt={nt})")
@@ -474,7 +495,7 @@ def main():
 
         log("bench: gradient workloads (true model = background + 5 anomalies)")
         true_full = synth.add_anomalies(m)
-        gradients = {"single_shot": gradient_single(m, args.grad_steps)}
+        gradients = {"single_shot": gradient_single(m, args.grad_steps, modes, gforms)}
         if args.grad_shots > 0:
             gradients["batched"] = gradient_batched(m, true_full, args.grad_steps, args.grad_shots)
 

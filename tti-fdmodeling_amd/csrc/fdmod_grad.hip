@@ -144,8 +144,8 @@ int store_io(fdmod_ctx *c, FieldStore &st, size_t slot, int nf, float *const *fi
   return FDMOD_OK;
 }
 
-// History slots hold the forward fields one step's imaging needs (n_hist fields,
-// two FP16 scale groups of scales_per_field each).
+// History slots hold the forward fields one step's imaging needs (n_hist fields; in
+// FP16, n_hist scale arrays of scales_per_field each).
 SweepArgs with_hist(fdmod_ctx *c, const SweepArgs &sa, size_t slot, bool f16)
 {
   SweepArgs r = sa;
@@ -157,7 +157,7 @@ SweepArgs with_hist(fdmod_ctx *c, const SweepArgs &sa, size_t slot, bool f16)
     else r.h32[f] = fld(c, (float *)c->hist.data, slot * nh + f);
   }
   if (f16)
-    for (int q = 0; q < 2; q++) r.hsc[q] = c->hist.scale + (slot * 2 + q) * ns;
+    for (int f = 0; f < nh; f++) r.hsc[f] = c->hist.scale + (slot * nh + f) * ns;
   return r;
 }
 
@@ -174,7 +174,8 @@ void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
     if (f16) b.h16[j][f] = fld(c, (const _Float16 *)c->hist.data, slot * nh + f);
     else b.h32[j][f] = fld(c, (const float *)c->hist.data, slot * nh + f);
   }
-  for (int q = 0; q < 2; q++) b.hsc[j][q] = f16 ? c->hist.scale + (slot * 2 + q) * ns : nullptr;
+  b.hsc[j] = f16 ? c->hist.scale + slot * nh * ns : nullptr;
+  b.hsc_stride = ns;
 }
 
 // Segment lengths of the two adjoint sweeps, chosen once per context.  The adjoint

@@ -70,18 +70,19 @@ struct SweepArgs {
   const uint8_t *undamped;  // per tile: coef1 == 1 and coef2 == dt everywhere in the tile
   int freesurface;
   // History output for the gradient, stored at every updated point as FP32 (estore 1)
-  // or as FP16 with one power-of-two scale per column, 64-row strip and group of
-  // fields (estore 2; scales [Nx][tilesZ] in hsc[group]):
-  //   Bond        stress sweep (vxdx, vzdz, vzdx + vxdz) -> h[0..2], group 0
-  //   tensor      stress sweep strain as for Bond -> h[0..2] (group 0) and the new tau
-  //               (before the stress source) -> h[3..5] (group 1)
-  //   coordinate  stress sweep (vxdx, vxdz, vzdx, vzdz) before rotation -> h[0..3]
-  //               (group 0); velocity sweep (-sxxdz + sxzdx, -sxzdz + szzdx,
-  //               sxxdx + sxzdz, sxzdx + szzdz) -> h[4..7] (group 1)
+  // or as FP16 with one power-of-two scale per field, column and 64-row strip
+  // (estore 2; scales [Nx][tilesZ] in hsc[field]):
+  //   Bond        stress sweep (vxdx, vzdz, vzdx + vxdz) -> h[0..2]
+  //   tensor      stress sweep (vxdx + vzdz, vxdx - vzdz, vzdx + vxdz) -> h[0..2] and
+  //               the new tau (before the stress source) as (txx + tzz, txx - tzz,
+  //               txz) -> h[3..5]
+  //   coordinate  stress sweep (vxdx, vxdz, vzdx, vzdz) before rotation -> h[0..3];
+  //               velocity sweep (-sxxdz + sxzdx, -sxzdz + szzdx, sxxdx + sxzdz,
+  //               sxzdx + szzdz) -> h[4..7]
   int estore;
   float *h32[8];
   _Float16 *h16[8];
-  float *hsc[2];
+  float *hsc[8];
   // Time-reversed runs (dt and coef2 negated by the caller): rev = 1 makes the
   // tensor-form history hold the stress sweep's input t, which is the forward's t
   // before the stress source once the caller has taken the source off.  FP32
@@ -155,15 +156,16 @@ struct AdjArgs {
 
 // One imaging launch over nk <= kImgK steps: the V-adjoint's snapshots of each step
 // (Bond 3, tensor 6, coordinate 5 fields) and the forward history of the step
-// (Bond 3, tensor 6, coordinate 8 fields; FP32 h32, or FP16 h16 with the per-(column,
-// strip) scales hsc of its two field groups).
+// (Bond 3, tensor 6, coordinate 8 fields; FP32 h32, or FP16 h16 with per-(field,
+// column, strip) scales: field f's scales of step j start at hsc[j] + f * hsc_stride).
 constexpr int kImgK = 16;
 struct ImgBlock {
   int nk;
   const float *lam[kImgK][6];
   const float *h32[kImgK][8];
   const _Float16 *h16[kImgK][8];
-  const float *hsc[kImgK][2];
+  const float *hsc[kImgK];
+  size_t hsc_stride;
 };
 
 // FP16 packing of nf padded fields: dst = src * 2^(14-E) per (column, 64-row block),

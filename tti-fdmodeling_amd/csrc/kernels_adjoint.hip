@@ -521,10 +521,8 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
 #pragma unroll
     for (int q = 0; q < NL; q++) L[q] = b.lam[j][q][o];
     if (F16) {
-      const float s0 = b.hsc[j][0][so], s1 = NH > 4 ? b.hsc[j][1][so] : 0.0f;
-      const int n0 = FORM == FDMOD_FORM_COORD ? 4 : 3;
 #pragma unroll
-      for (int q = 0; q < NH; q++) H[q] = (float)b.h16[j][q][o] * (q < n0 ? s0 : s1);
+      for (int q = 0; q < NH; q++) H[q] = (float)b.h16[j][q][o] * b.hsc[j][q * b.hsc_stride + so];
     } else {
 #pragma unroll
       for (int q = 0; q < NH; q++) H[q] = b.h32[j][q][o];
@@ -539,7 +537,9 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
       acc[5] += Y * eg;
     } else if (FORM == FDMOD_FORM_TENSOR) {
       const double lx = L[0], lz = L[1], lc = L[2], a1 = L[3], a2 = L[4], a3 = L[5];
-      const double d = H[0], z = H[1], gg = H[2], t1 = H[3], t2 = H[4], t3 = H[5];
+      // the history holds (vxdx + vzdz, vxdx - vzdz, shear, txx + tzz, txx - tzz, txz)
+      const double d = 0.5 * (H[0] + H[1]), z = 0.5 * (H[0] - H[1]), gg = H[2];
+      const double t1 = 0.5 * (H[3] + H[4]), t2 = 0.5 * (H[3] - H[4]), t3 = H[5];
       const double k2 = P[4], S2 = P[5], K2 = P[6], S2T = P[7];
       const double exx = K2 * d + S2 * z - 0.5 * S2T * gg, ezz = S2 * d + K2 * z + 0.5 * S2T * gg;
       const double exz = S2T * (d - z) + (K2 - S2) * gg;

@@ -20,11 +20,13 @@
 
 namespace fdm {
 
-// Store one column of a group of history fields for the gradient (all lanes call it;
-// rows past izEnd pass zeros).  FP16: one scale for the group's fields.
+// Store one column of NF history fields f0 .. f0+NF-1 for the gradient (all lanes call
+// it; rows past izEnd pass zeros).  FP16: each field is stored as v * 2^(14-E) with E
+// the exponent of its largest |v| in the column's 64-row strip (|stored| < 2^14), and
+// the strip keeps 2^(E-14) for that field.
 template <int ESTORE, int NF>
 __device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, int btz, bool zok, const float (&e)[NF],
-                                           int f0, int grp)
+                                           int f0)
 {
   if (ESTORE == 1) {
     const int iz = (int)(o - (size_t)c * a.g.ldz);
@@ -34,18 +36,15 @@ __device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, 
       for (int f = 0; f < NF; f++) a.h32[f0 + f][o] = e[f];
     return;
   }
-  float m = 0.0f;
 #pragma unroll
-  for (int f = 0; f < NF; f++) m = fmaxf(m, fabsf(e[f]));
-  m = wave_max64(m);
-  int E = 0;
-  if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
-  E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
-  const float up = ldexpf(1.0f, 14 - E);
-  if (zok)
-#pragma unroll
-    for (int f = 0; f < NF; f++) a.h16[f0 + f][o] = (_Float16)(e[f] * up);
-  if (threadIdx.x == 0) a.hsc[grp][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
+  for (int f = 0; f < NF; f++) {
+    const float m = wave_max64(fabsf(e[f]));
+    int E = 0;
+    if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
+    E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
+    if (zok) a.h16[f0 + f][o] = (_Float16)(e[f] * ldexpf(1.0f, 14 - E));
+    if (threadIdx.x == 0) a.hsc[f0 + f][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
+  }
 }
 
 // ---------------------------------------------------------------- velocity sweep
@@ -150,7 +149,7 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
           }
         }
       }
-      if (ESTORE) store_hist<ESTORE, 4>(a, (size_t)c * g.ldz + iz, c, btz, zok, hv, 4, 1);
+      if (ESTORE) store_hist<ESTORE, 4>(a, (size_t)c * g.ldz + iz, c, btz, zok, hv, 4);
 #pragma unroll
       for (int i = 0; i + 1 < PD; i++) {
         colq[i] = colq[i + 1];
@@ -282,9 +281,16 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
         rsg_bwd([&](int dx, int dz) { return ring[1][slot(dx)][lane + 5 + dz]; }, d1, d2);
         float vzdx = 0.5f * (d1 + d2) * invdx;
         float vzdz = 0.5f * (d1 - d2) * invdz;
-        if (ESTORE && FORM != FDMOD_FORM_COORD) {
+        if (ESTORE && FORM == FDMOD_FORM_BOND) {
           hv[0] = vxdx;
           hv[1] = vzdz;
+          hv[2] = vzdx + vxdz;
+        } else if (ESTORE && FORM == FDMOD_FORM_TENSOR) {
+          // sum and difference of the normal strains: the theta imaging uses their
+          // difference, which the stored precision then carries directly (for
+          // P-dominated wavefields vxdx and vzdz nearly cancel there)
+          hv[0] = vxdx + vzdz;
+          hv[1] = vxdx - vzdz;
           hv[2] = vzdx + vxdz;
         } else if (ESTORE) {
           hv[0] = vxdx;
@@ -320,8 +326,9 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
           st_out(a.txx + o, t1);
           st_out(a.tzz + o, t2);
           st_out(a.txz + o, t3);
-          hv[3] = a.rev ? op.s1 : t1;
-          hv[4] = a.rev ? op.s2 : t2;
+          // tau stored as (txx + tzz, txx - tzz, txz), for the same reason
+          hv[3] = a.rev ? op.s1 + op.s2 : t1 + t2;
+          hv[4] = a.rev ? op.s1 - op.s2 : t1 - t2;
           hv[5] = a.rev ? op.s3 : t3;
           nxx = K2 * t1 + S2 * t2 + S2T * t3;
           nzz = S2 * t1 + K2 * t2 - S2T * t3;
@@ -405,14 +412,14 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
       if (ESTORE) {
         if (FORM == FDMOD_FORM_BOND) {
           const float e[3] = {hv[0], hv[1], hv[2]};
-          store_hist<ESTORE, 3>(a, o, c, btz, zok, e, 0, 0);
+          store_hist<ESTORE, 3>(a, o, c, btz, zok, e, 0);
         } else if (FORM == FDMOD_FORM_TENSOR) {
           const float e[3] = {hv[0], hv[1], hv[2]}, t[3] = {hv[3], hv[4], hv[5]};
-          store_hist<ESTORE, 3>(a, o, c, btz, zok, e, 0, 0);
-          store_hist<ESTORE, 3>(a, o, c, btz, zok, t, 3, 1);
+          store_hist<ESTORE, 3>(a, o, c, btz, zok, e, 0);
+          store_hist<ESTORE, 3>(a, o, c, btz, zok, t, 3);
         } else {
           const float e[4] = {hv[0], hv[1], hv[2], hv[3]};
-          store_hist<ESTORE, 4>(a, o, c, btz, zok, e, 0, 0);
+          store_hist<ESTORE, 4>(a, o, c, btz, zok, e, 0);
         }
       }
 #pragma unroll
