@@ -254,6 +254,19 @@ int fdmod_adjoint_dot(fdmod_ctx *ctx, const float *y_vx, const float *y_vz, cons
  * reusing the context. */
 int fdmod_time_sweeps(fdmod_ctx *ctx, int reps, double *ms_v, double *ms_s);
 
+/* ---- Allocation-free run (no counterpart in the reference; for callers that cannot
+ * allocate device memory: code under HIP graph capture, serving loops that must not call
+ * hipMalloc).  Runs loop iterations k0 .. k0+count-1 as fdmod_run does and writes the
+ * energy diagnostic of each (BT:926-952; the values fdmod_set_energy + fdmod_run +
+ * fdmod_get_energy give) to E/Ek/Ep[j] for iteration k0 + j.  It calls no hipMalloc or
+ * hipFree: its device scratch (the energy kernels' partial sums and the series of the
+ * call) is one static array per GPU, held until the process ends.  At most
+ * FDMOD_NOALLOC_MAX_STEPS iterations per call (longer runs: several calls); snapshots are
+ * refused (wsave must be 0); the context's own energy series is not advanced.  Calls that
+ * overlap on one GPU share that array and race. */
+#define FDMOD_NOALLOC_MAX_STEPS 16384
+int fdmod_run_energy_noalloc(fdmod_ctx *ctx, int k0, int count, float *E, float *Ek, float *Ep, double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,18 @@ class Modeling:
     def reset(self):
         self._check(self.L.fdmod_reset(self.c))
 
+    def run_energy_noalloc(self, k0=0, count=None):
+        """Run loop iterations k0 .. k0+count-1 through the allocation-free entry point;
+        returns the energy series (E, Ek, Ep) of those iterations and the device seconds."""
+        if count is None:
+            count = self.nt - k0
+        n = max(0, min(k0 + count, self.nt) - k0)
+        E, Ek, Ep = (np.zeros(max(n, 1), dtype=np.float32) for _ in range(3))
+        sec = ct.c_double(0)
+        self._check(self.L.fdmod_run_energy_noalloc(self.c, k0, count, _ptr(E, ct.c_float), _ptr(Ek, ct.c_float),
+                                                    _ptr(Ep, ct.c_float), ct.byref(sec)))
+        return E[:n], Ek[:n], Ep[:n], sec.value
+
     def traces(self, comp):
         """comp: 1 = vx, 3 = vz, 4 = p (traces2d_output flags)."""
         out = np.empty(self.nr * self.ntsample, dtype=np.float32)

@@ -55,6 +55,7 @@ SIGNATURES = {
     "fdmod_set_snapshots": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, SNAPSHOT_FN, ct.c_void_p]),
     "fdmod_set_energy": (ct.c_int, [ct.c_void_p, ct.c_int]),
     "fdmod_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _dp]),
+    "fdmod_run_energy_noalloc": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _fp, _fp, _fp, _dp]),
     "fdmod_reset": (ct.c_int, [ct.c_void_p]),
     "fdmod_get_traces": (ct.c_int, [ct.c_void_p, ct.c_int, _fp]),
     "fdmod_get_energy": (ct.c_int, [ct.c_void_p, _fp, _fp, _fp]),
