@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BUILD = os.path.join(HERE, "_build")
+# ORACLE_BUILD_DIR selects another build of the same sources (the sanitizer build)
+BUILD = os.environ.get("ORACLE_BUILD_DIR") or os.path.join(HERE, "_build")
 
 FORMS = {"coord": 0, "tensor": 1, "bond": 2}
 ARRAYS = ["vp", "vs", "rho", "eps", "delta", "theta",

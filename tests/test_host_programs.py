@@ -12,7 +12,10 @@ import sufile
 from fdmod_hip import synth
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "tti-fdmodeling_amd", "bin")
+# FDMOD_HOST_BIN / FDMOD_HOST_LIB select other builds of the host programs and helpers
+# (the sanitizer build of oracle/sanitize_check.sh)
+BIN = os.environ.get("FDMOD_HOST_BIN") or os.path.join(ROOT, "tti-fdmodeling_amd", "bin")
+HOSTLIB = os.environ.get("FDMOD_HOST_LIB") or os.path.join(ROOT, "tti-fdmodeling_amd", "_build", "libfdmod_host.so")
 PROGS = ["sufdmod2dTTIelastic_bond_transformation", "sufdmod2dTTIelastic_tensor_rotation",
          "sufdmod2dTTIelastic_coordinate_rotation"]
 
@@ -227,7 +230,7 @@ def test_program_reference_diagnostics(tmp_path):
 def _resample(y, din, fin, t):
     import ctypes as ct
 
-    lib = ct.CDLL(os.path.join(ROOT, "tti-fdmodeling_amd", "_build", "libfdmod_host.so"))
+    lib = ct.CDLL(HOSTLIB)
     fp = ct.POINTER(ct.c_float)
     lib.resample8.argtypes = [ct.c_int, ct.c_float, ct.c_float, fp, ct.c_int, fp, fp]
     y = np.ascontiguousarray(y, np.float32)
@@ -239,9 +242,9 @@ def _resample(y, din, fin, t):
 
 @pytest.fixture(scope="module")
 def hostlib():
-    path = os.path.join(ROOT, "tti-fdmodeling_amd", "_build", "libfdmod_host.so")
-    if not os.path.exists(path):
-        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tti-fdmodeling_amd"), path], check=True)
+    if not os.path.exists(HOSTLIB):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tti-fdmodeling_amd"), "_build/libfdmod_host.so"],
+                       check=True)
 
 
 @pytest.mark.parametrize("d", [0.125, 185 / 512, 0.5, 0.875])
