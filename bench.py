@@ -362,6 +362,63 @@ def gradient_batched(full, true_full, nt, nshots, seed=5):
             "batched_vs_plain_rel_l2": [round(x, 7) for x in _rel(gb, gp)]}
 
 
+def cpu_gradient_c4(nt=4000):
+    """The FP64 oracle gradient on the gradient workload's own geometry (1126 x 2000 crop,
+    nt = 4000, Bond form, checkpointed recomputation, OpenMP on the host cores) and the GPU
+    gradients' relative L2 errors against it, per parameter (vp, eps, delta, theta), with
+    the observed data shared (modelled by the FP64 oracle) and with each path's own data
+    (the GPU's FP32 modelling for the GPU gradients); the FP32 oracle (the reference's
+    arithmetic) is measured the same way."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from fdmod_hip import Modeling, synth
+
+    host = host_cpu()
+    cores = int(os.environ.get("FDMOD_CPU_THREADS", "0")) or host["physical_cores"]
+    os.environ.setdefault("ORACLE_PROGRESS", "500")  # the FP64 legs run for minutes each
+    oracle.set_threads(cores)
+    oracle.set_threads(cores, "f64")
+    W, x0 = 2000, 2936
+    bg = synth.crop(synth.model_large(), x0, W)
+    tr = synth.add_anomalies(bg)
+    log("bench: FP64 oracle forward on the true model (1126 x 2000, nt = %d)" % nt)
+    t0 = time.perf_counter()
+    ot = _grad_shot(oracle.Oracle, tr, W, W // 2, nt, prec="f64")
+    ot.run()
+    obs64 = ot.traces(4).astype(np.float32)
+    ot.close()
+    log("bench: FP64 oracle gradient")
+    o = _grad_shot(oracle.Oracle, bg, W, W // 2, nt, prec="f64")
+    J64, g64 = o.gradient(obs_p=obs64)
+    o.close()
+    w64 = time.perf_counter() - t0
+    log("bench: FP32 oracle gradients (shared and own data)")
+    o32 = _grad_shot(oracle.Oracle, bg, W, W // 2, nt, prec="f32")
+    _, g32 = o32.gradient(obs_p=obs64)
+    ot32 = _grad_shot(oracle.Oracle, tr, W, W // 2, nt, prec="f32")
+    ot32.run()
+    _, g32o = o32.gradient(obs_p=ot32.traces(4).astype(np.float32))
+    ot32.close()
+    o32.close()
+    errs = {"reference_fp32_oracle": [round(x, 7) for x in _rel(g32, g64)]}
+    own = {"reference_fp32_oracle": [round(x, 7) for x in _rel(g32o, g64)]}
+    gt = _grad_shot(Modeling, tr, W, W // 2, nt)
+    gt.run()
+    obs_gpu = gt.traces(4).copy()
+    gt.close()
+    g = _grad_shot(Modeling, bg, W, W // 2, nt)
+    for name in ("plain_f32_checkpoints", "f16_history"):
+        _, gg = g.gradient(obs_p=obs64, **GRAD_MODES[name])
+        errs[name] = [round(x, 7) for x in _rel(gg, g64)]
+        _, gg = g.gradient(obs_p=obs_gpu, **GRAD_MODES[name])
+        own[name] = [round(x, 7) for x in _rel(gg, g64)]
+    g.close()
+    return {"sample": f"FP64 oracle gradient on the 1126 x {W} crop, nt={nt}, Bond form, no free surface, true model "
+                      "= background + 5 anomalies inside the crop; ceil(sqrt(nt)) checkpoints, OpenMP adjoint",
+            "seconds_fp64_forward_and_gradient": round(w64, 1), "cores": cores, "misfit_fp64": J64,
+            "gpu_rel_l2_vs_fp64": errs, "gpu_rel_l2_vs_fp64_own_data": own}
+
+
 def cpu_gradient_replica(seed=7):
     """FP64 oracle gradient (the CPU reference for the adjoint) on a reduced replica of
     the gradient workload, timed, and the GPU FP32 and FP16 gradients on the same
@@ -442,6 +499,8 @@ def main():
     ap.add_argument("--grad-only", action="store_true", help="only the single-shot gradient workload (profiling)")
     ap.add_argument("--grad-modes", default=",".join(GRAD_MODES), help="single-shot gradient modes (comma list)")
     ap.add_argument("--grad-forms", default="bond,tensor,coord", help="forms of the single-shot gradient workload")
+    ap.add_argument("--fp64-c4", action="store_true",
+                    help="with --grad-only: the FP64 oracle gradient on the 1126 x 2000 workload (minutes of CPU)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU shot sharding is a design note only",
@@ -456,7 +515,10 @@ def main():
 
         log("bench: gradient workload only")
         full = synth.model_large()
-        print(json.dumps({"gradients": {"single_shot": gradient_single(full, args.grad_steps, modes, gforms)}}))
+        out = {"single_shot": gradient_single(full, args.grad_steps, modes, gforms)}
+        if args.fp64_c4:
+            out["cpu_fp64_c4"] = cpu_gradient_c4(args.grad_steps)
+        print(json.dumps({"gradients": out}))
         return
     nt = max(args.steps, args.cpu_steps, 1)
     log(f"bench: building the 1126x7872 workload (nt={nt})")

@@ -606,6 +606,21 @@ static void step(octx *c, int it)
   if (c->energy) energy(c, it);
 }
 
+/* Progress lines on stderr every `every` steps when ORACLE_PROGRESS is set (long
+ * FP64 runs that would otherwise print nothing for minutes). */
+static void progress(const char *what, int k, int nt)
+{
+  static int every = -1;
+  if (every < 0) {
+    const char *env = getenv("ORACLE_PROGRESS");
+    every = env ? atoi(env) : 0;
+  }
+  if (every > 0 && k % every == 0) {
+    fprintf(stderr, "oracle: %s step %d of %d\n", what, k, nt);
+    fflush(stderr);
+  }
+}
+
 /* Run loop iterations k in [k0, k0+count); it = k (mode 1) or nt-1-k (mode 2),
  * BT:651.  Returns the loop wall time in seconds (omp_get_wtime, BT:650,955). */
 double PFX(run)(octx *c, int k0, int count)
@@ -613,7 +628,10 @@ double PFX(run)(octx *c, int k0, int count)
 #ifdef _OPENMP
   double t0 = omp_get_wtime();
 #endif
-  for (int k = k0; k < k0 + count && k < c->nt; k++) step(c, c->mode == 1 ? k : c->nt - 1 - k);
+  for (int k = k0; k < k0 + count && k < c->nt; k++) {
+    progress("forward", k, c->nt);
+    step(c, c->mode == 1 ? k : c->nt - 1 - k);
+  }
 #ifdef _OPENMP
   return omp_get_wtime() - t0;
 #else
