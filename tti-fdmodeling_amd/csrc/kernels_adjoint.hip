@@ -122,12 +122,14 @@ struct VAdjOps {
 // sweep), p in U.  Bond/tensor: (Dx M0 + Dz M2, Dx M2 + Dz M1); coordinate:
 // (Dx M0 + Dz M1, Dx M2 + Dz M3).
 // Staged column of the stress adjoint: lane l holds row rs + l only (the strip's 64
-// staged rows are its kTZS = 55 output rows and their 4 + 5 halo rows).
+// staged rows are its kTZS = 55 output rows and their 4 + 5 halo rows).  The caller
+// passes the row clamped into the grid: rows past the grid feed no output row.  No
+// branch surrounds the loads, so that the look-ahead stays in flight (a load under a
+// branch makes the compiler wait for every load before the next use of an older one).
 template <int FORM>
-__device__ __forceinline__ void load_mrow_lane(MRow &r, const AdjArgs &a, int gx, int row, const Grid &g, bool und)
+__device__ __forceinline__ void load_mrow_lane(MRow &r, const AdjArgs &a, int gx, int rowc, const Grid &g, bool und)
 {
-  r = MRow{};
-  if (row >= 0 && row < g.Nz) load_mrow<FORM>(r, a, (size_t)gx * g.ldz + row, und);
+  load_mrow<FORM>(r, a, (size_t)gx * g.ldz + rowc, und);
 }
 
 template <int FORM>
@@ -156,6 +158,8 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   const int nc = min(segL, g.ixEnd - xa);
   const int rs = z0 - 4;
   const int iz = rs + lane;
+  const int izc = min(max(iz, 0), g.Nz - 1);  // in-grid row for the loads
+  const int xe = xa + nc - 1;
   const bool zok = lane >= 4 && lane < 4 + kTZS && iz < g.izEnd;
   // staged rows rs .. rs+63 of columns xa-4 .. xa+nc+4
   const bool und = region_undamped(a, rs, rs + TZ, xa - 4, xa + nc + 5);
@@ -164,22 +168,23 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   for (int m0 = 0; m0 < 9; m0 += 3) {
     MRow c[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) load_mrow_lane<FORM>(c[m], a, xa - 4 + m0 + m, iz, g, und);
+    for (int m = 0; m < 3; m++) load_mrow_lane<FORM>(c[m], a, xa - 4 + m0 + m, izc, g, und);
 #pragma unroll
     for (int m = 0; m < 3; m++) store_mrow<FORM>(ring, m0 + m, c[m], lane);
   }
   MRow colq[PD];
   VAdjOps opq[PD];
   auto load_ops = [&](VAdjOps &p, int col) {
-    const size_t o = (size_t)col * g.ldz + iz;
+    const size_t o = (size_t)col * g.ldz + izc;
     p.mvx = a.mvx[o];
     p.mvz = a.mvz[o];
     p.k1 = und ? 1.0f : a.coef1[o];
   };
+  // look-ahead loads from clamped columns (values past the segment are not used)
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    if (i <= nc - 1) load_mrow_lane<FORM>(colq[i], a, xa + 5 + i, iz, g, und);
-    if (zok && i < nc) load_ops(opq[i], xa + i);
+    load_mrow_lane<FORM>(colq[i], a, min(xa + 5 + i, xe + 5), izc, g, und);
+    load_ops(opq[i], min(xa + i, xe));
   }
   const float invdx = g.invdx, invdz = g.invdz;
   for (int k0 = 0; k0 < nc; k0 += kRing) {
@@ -190,8 +195,8 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
       const int c = xa + k;
       MRow cn;
       VAdjOps on;
-      if (k <= nc - PD - 1) load_mrow_lane<FORM>(cn, a, c + 5 + PD, iz, g, und);
-      if (zok && k + PD < nc) load_ops(on, c + PD);
+      load_mrow_lane<FORM>(cn, a, min(c + 5 + PD, xe + 5), izc, g, und);
+      load_ops(on, min(c + PD, xe));
       store_mrow<FORM>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
       if (zok) {
@@ -250,15 +255,15 @@ __device__ __forceinline__ void load_nrow(NRow &r, const AdjArgs &a, size_t o, b
   }
 }
 
+// Staged column rows rs+lane (lo) and, for lanes < kHalo, rs+64+lane (hi), loaded with no
+// branch around them from rows clamped into the grid (r0c, r1c); store_ncol zeroes the
+// rows outside the grid (ok0, ok1).
 template <int FORM>
-__device__ __forceinline__ void load_ncol(NCol &c, const AdjArgs &a, int gx, int rs, int lane, const Grid &g, bool und)
+__device__ __forceinline__ void load_ncol(NCol &c, const AdjArgs &a, int gx, int r0c, int r1c, const Grid &g, bool und)
 {
   const size_t base = (size_t)gx * g.ldz;
-  const int r0 = rs + lane, r1 = rs + TZ + lane;
-  c.lo = NRow{0, 0, 0, 0, 0, 0};
-  c.hi = NRow{0, 0, 0, 0, 0, 0};
-  if (r0 >= 0 && r0 < g.Nz) load_nrow<FORM>(c.lo, a, base + r0, und);
-  if (lane < kHalo && r1 >= 0 && r1 < g.Nz) load_nrow<FORM>(c.hi, a, base + r1, und);
+  load_nrow<FORM>(c.lo, a, base + r0c, und);
+  load_nrow<FORM>(c.hi, a, base + r1c, und);
 }
 
 template <int FORM>
@@ -286,22 +291,26 @@ __device__ __forceinline__ void n_of(const NRow &r, float (&n)[6])
 }
 
 template <int FORM>
-__device__ __forceinline__ void store_ncol(float (*ring)[kRing][LZ], int slot, const NCol &c, int lane)
+__device__ __forceinline__ void store_ncol(float (*ring)[kRing][LZ], int slot, const NCol &c, int lane, bool ok0,
+                                           bool ok1)
 {
   constexpr int NN = n_fields<FORM>();
   float n[6];
   n_of<FORM>(c.lo, n);
 #pragma unroll
-  for (int f = 0; f < NN; f++) ring[f][slot][lane] = n[f];
+  for (int f = 0; f < NN; f++) ring[f][slot][lane] = ok0 ? n[f] : 0.0f;
   if (lane < kHalo) {
     n_of<FORM>(c.hi, n);
 #pragma unroll
-    for (int f = 0; f < NN; f++) ring[f][slot][TZ + lane] = n[f];
+    for (int f = 0; f < NN; f++) ring[f][slot][TZ + lane] = ok1 ? n[f] : 0.0f;
   }
 }
 
+// Per-point operands of the velocity adjoint as loaded; anything computed from them is
+// computed where they are used, so that no instruction waits on a look-ahead load in the
+// iteration that issues it.
 struct SAdjOps {
-  float s1, s2, s3, k1, t1, t2, t3, S2, K2, S2T, na, nb;
+  float s1, s2, s3, k1, t1, t2, t3, S2, K2, S2T, c2, ir, mvx, mvz;
 };
 
 // mu_sigma(q) <- k1(q) mu_sigma(q) - (stress-gradient adjoints)(q), with k1 = coef1 in U
@@ -325,43 +334,52 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
   const bool zok = iz < g.izEnd;
   const bool inU = iz >= a.izU;
   const int rs = z0 - 5;
+  const int xe = xa + nc - 1;
+  const int izc = min(iz, g.Nz - 1);  // in-grid row for the per-point loads
+  // staged rows rs+lane and rs+64+lane, clamped into the grid for the loads
+  const int r0 = rs + lane, r1 = rs + TZ + lane;
+  const bool ok0 = r0 >= 0 && r0 < g.Nz, ok1 = lane < kHalo && r1 >= 0 && r1 < g.Nz;
+  const int r0c = min(max(r0, 0), g.Nz - 1), r1c = ok1 ? r1 : r0c;
   // staged rows rs .. rs+72 of columns xa-5 .. xa+nc+3
   const bool und = region_undamped(a, rs, rs + LZ, xa - 5, xa + nc + 4);
 #pragma unroll
   for (int m0 = 0; m0 < 9; m0 += 3) {
     NCol c[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) load_ncol<FORM>(c[m], a, xa - 5 + m0 + m, rs, lane, g, und);
+    for (int m = 0; m < 3; m++) load_ncol<FORM>(c[m], a, xa - 5 + m0 + m, r0c, r1c, g, und);
 #pragma unroll
-    for (int m = 0; m < 3; m++) store_ncol<FORM>(ring, m0 + m, c[m], lane);
+    for (int m = 0; m < 3; m++) store_ncol<FORM>(ring, m0 + m, c[m], lane, ok0, ok1);
   }
   NCol colq[PD];
   SAdjOps opq[PD];
+  // per-point operands, loaded with no branch around them (selected where used)
   auto load_ops = [&](SAdjOps &p, int col) {
-    const size_t o = (size_t)col * g.ldz + iz;
+    const size_t o = (size_t)col * g.ldz + izc;
     p = SAdjOps{};
     p.s1 = a.mxx[o];
     p.s2 = a.mzz[o];
     p.s3 = a.mxz[o];
-    p.k1 = inU && !und ? a.coef1[o] : 1.0f;
+    p.k1 = und ? 1.0f : a.coef1[o];
     if (FORM == FDMOD_FORM_TENSOR) {
       p.t1 = a.mtxx[o];
       p.t2 = a.mtzz[o];
       p.t3 = a.mtxz[o];
-      p.S2 = inU ? a.sin2[o] : 0.0f;
-      p.K2 = inU ? a.cos2[o] : 0.0f;
-      p.S2T = inU ? a.sin2t[o] : 0.0f;
+      p.S2 = a.sin2[o];
+      p.K2 = a.cos2[o];
+      p.S2T = a.sin2t[o];
     }
-    if (FORM == FDMOD_FORM_COORD && a.snap[3] && inU) {
-      const float kr = (und ? g.dt : a.coef2[o]) * a.invrho[o];
-      p.na = kr * a.mvx[o];
-      p.nb = kr * a.mvz[o];
+    if (FORM == FDMOD_FORM_COORD && a.snap[3]) {
+      p.c2 = und ? g.dt : a.coef2[o];
+      p.ir = a.invrho[o];
+      p.mvx = a.mvx[o];
+      p.mvz = a.mvz[o];
     }
   };
+  // look-ahead loads from clamped columns (values past the segment are not used)
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    if (i <= nc - 1) load_ncol<FORM>(colq[i], a, xa + 4 + i, rs, lane, g, und);
-    if (zok && i < nc) load_ops(opq[i], xa + i);
+    load_ncol<FORM>(colq[i], a, min(xa + 4 + i, xe + 4), r0c, r1c, g, und);
+    load_ops(opq[i], min(xa + i, xe));
   }
   const float invdx = g.invdx, invdz = g.invdz;
   for (int k0 = 0; k0 < nc; k0 += kRing) {
@@ -372,9 +390,9 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
       const int c = xa + k;
       NCol cn;
       SAdjOps on;
-      if (k <= nc - PD - 1) load_ncol<FORM>(cn, a, c + 4 + PD, rs, lane, g, und);
-      if (zok && k + PD < nc) load_ops(on, c + PD);
-      store_ncol<FORM>(ring, (u + 9) % kRing, colq[0], lane);
+      load_ncol<FORM>(cn, a, min(c + 4 + PD, xe + 4), r0c, r1c, g, und);
+      load_ops(on, min(c + PD, xe));
+      store_ncol<FORM>(ring, (u + 9) % kRing, colq[0], lane, ok0, ok1);
       __syncthreads();
       if (zok) {
         // d1, d2 of staged field f; then x- and z-derivative forms
@@ -385,6 +403,7 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
                   dd2[f]);
         const size_t o = (size_t)c * g.ldz + iz;
         const SAdjOps &op = opq[0];
+        const float k1 = inU ? op.k1 : 1.0f;  // 1 on the image rows above U
         float gxx, gzz, gxz;  // adjoint contributions to subtract
         if constexpr (FORM != FDMOD_FORM_COORD) {
           const float nxdx = 0.5f * (dd1[0] + dd2[0]) * invdx, nxdz = 0.5f * (dd1[0] - dd2[0]) * invdz;
@@ -402,9 +421,9 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
           const float a1 = op.K2 * op.s1 + op.S2 * op.s2 - 0.5f * op.S2T * op.s3 + op.t1;
           const float a2 = op.S2 * op.s1 + op.K2 * op.s2 + 0.5f * op.S2T * op.s3 + op.t2;
           const float a3 = op.S2T * op.s1 - op.S2T * op.s2 + (op.K2 - op.S2) * op.s3 + op.t3;
-          a.mtxx[o] = op.k1 * a1;
-          a.mtzz[o] = op.k1 * a2;
-          a.mtxz[o] = op.k1 * a3;
+          a.mtxx[o] = k1 * a1;
+          a.mtzz[o] = k1 * a2;
+          a.mtxz[o] = k1 * a3;
           a.oxx[o] = -gxx;
           a.ozz[o] = -gzz;
           a.oxz[o] = -gxz;
@@ -414,12 +433,13 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
             a.snap[5][o] = a3;
           }
         } else {
-          a.oxx[o] = op.k1 * op.s1 - gxx;
-          a.ozz[o] = op.k1 * op.s2 - gzz;
-          a.oxz[o] = op.k1 * op.s3 - gxz;
+          a.oxx[o] = k1 * op.s1 - gxx;
+          a.ozz[o] = k1 * op.s2 - gzz;
+          a.oxz[o] = k1 * op.s3 - gxz;
           if (FORM == FDMOD_FORM_COORD && a.snap[3] && inU) {
-            a.snap[3][o] = op.na;
-            a.snap[4][o] = op.nb;
+            const float kr = op.c2 * op.ir;
+            a.snap[3][o] = kr * op.mvx;
+            a.snap[4][o] = kr * op.mvz;
           }
         }
       }
