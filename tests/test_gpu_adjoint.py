@@ -149,6 +149,42 @@ def test_gradient_storage_modes(form):
         assert max(errs) <= (2e-3 if storage == "f16all" else 1e-3), errs
 
 
+# Pressure receivers on one row at consecutive columns (the gradient workloads' geometry):
+# the two adjoint sweeps and the imaging add the pressure residuals themselves, with no
+# injection launch.  The same receivers listed right to left take the separate injection
+# kernel; both gradients must agree to rounding, and with the FP64 oracle.
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("storage,hist", [("f32", False), ("f16", True)])
+def test_fused_line_injection_matches_separate_injection(form, storage, hist):
+    print("fused injection", form, storage, hist, flush=True)
+    m = _model()
+    true = {k: v.copy() for k, v in m.items()}
+    true["vp"] = (true["vp"] * 1.03).astype(np.float32)
+    true["theta"] = (true["theta"] + 4).astype(np.float32)
+    rx, rz = np.arange(NX), np.full(NX, 3)
+
+    def mk(cls, model, xs, **kw):
+        o = cls(form, NZ, NX, NT, 10.0, 10.0, DT, ealThick=10, freesurface=0, fpeak=15.0, **kw)
+        o.set_model(model["vp"], model["vs"], model["rho"], model["eps"], model["delta"], model["theta"])
+        o.set_source(1, [30], [6], synth.ricker(15.0, DT, NT)[None])
+        o.set_receivers(2, 1, xs, rz)
+        return o
+
+    ot = mk(Oracle, true, rx, prec="f64")
+    ot.run()
+    obs = ot.traces(4).astype(np.float32)
+    J1, g1 = mk(Modeling, m, rx).gradient(None, None, obs, storage=storage, history=hist)
+    J2, g2 = mk(Modeling, m, rx[::-1].copy()).gradient(None, None, obs[::-1].copy(), storage=storage, history=hist)
+    assert abs(J1 - J2) <= 1e-12 * J1, (J1, J2)
+    for i in range(4):
+        assert rel_l2(g1[i], g2[i]) <= 1e-12, (i, rel_l2(g1[i], g2[i]))
+    Jr, gr = mk(Oracle, m, rx, prec="f64").gradient(None, None, obs)
+    assert abs(J1 - Jr) <= 1e-5 * Jr
+    errs = [rel_l2(g1[i], gr[i]) for i in range(4)]
+    print("  vs FP64", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) <= (1e-4 if storage == "f32" else 1e-3), errs
+
+
 def test_gradient_finite_differences_16_directions():
     m, obs = _obs(0, 2)
     g = _mk(Modeling, m, 0, seis=2)

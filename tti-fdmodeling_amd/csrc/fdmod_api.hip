@@ -397,6 +397,7 @@ extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, c
   c->seistype = seistype;
   c->mt = mt;
   c->nr = nr;
+  c->rline = false;
   c->ntsample = (p.nt - 1) / mt + 1;  // obsinfo.c:199
   if (nr == 0 || seistype == 0) return FDMOD_OK;
   std::vector<int> gx(nr), gz(nr);
@@ -418,6 +419,14 @@ extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, c
   if (seistype == 2 || seistype == 3) {
     HIPCALL(c, hipMalloc(&c->d_rp, nrec));
     HIPCALL(c, hipMemsetAsync(c->d_rp, 0, nrec, c->stream));
+  }
+  if (seistype == 2) {
+    // one row, consecutive columns in receiver order (each receiver its own cell)
+    bool line = true;
+    for (int i = 1; i < nr && line; i++) line = gz[i] == gz[0] && gx[i] == gx[0] + i;
+    c->rline = line;
+    c->rline_x0 = gx[0];
+    c->rline_z = gz[0];
   }
   if (seistype == 2) {  // pressure receivers by padded column, for the stress sweep to record
     const int Nx = c->g.Nx;

@@ -42,6 +42,11 @@ struct fdmod_ctx {
   int rc_zlo = 0, rc_zhi = -1;  // padded rows spanned by the receivers
   int *d_gx = nullptr, *d_gz = nullptr;
   float *d_rvx = nullptr, *d_rvz = nullptr, *d_rp = nullptr;
+  // pressure receivers i = 0..nr-1 at padded cell (rline_x0 + i, rline_z): the adjoint
+  // sweeps then add the pressure residuals themselves (no injection launch)
+  bool rline = false;
+  int rline_x0 = 0, rline_z = 0;
+  const float *inj_last = nullptr;  // the last adjoint step's fused residuals (or null)
   // snapshots
   int wsave = 0, savew = 0, snap_sample = 1;
   fdmod_snapshot_fn snap_fn = nullptr;

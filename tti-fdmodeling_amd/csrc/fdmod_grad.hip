@@ -176,6 +176,10 @@ void add_to_block(fdmod_ctx *c, ImgBlock &b, int j, size_t slot, bool f16)
   }
   b.hsc[j] = f16 ? c->hist.scale + slot * nh * ns : nullptr;
   b.hsc_stride = ns;
+  b.inj[j] = c->inj_last;  // set by the adjoint step just run
+  b.inj_x0 = c->rline_x0;
+  b.inj_z = c->rline_z;
+  b.inj_n = c->nr;
 }
 
 // Segment lengths of the two adjoint sweeps, chosen once per context.  The adjoint
@@ -285,7 +289,9 @@ int upload_data(fdmod_ctx *c, const float *hvx, const float *hvz, const float *h
 // One adjoint step: the transpose of loop iteration k.  The V-adjoint moves mu_sigma to
 // the next ring slot; the slot it read (mus_last) holds this step's imaged mu_sigma.
 // snap_slot >= 0: it also stores the other imaged fields (tensor and coordinate forms)
-// into that snapshot slot.
+// into that snapshot slot, and the step is imaged: pressure receivers on one row of
+// consecutive columns then have their residuals added by the sweeps and the imaging
+// (c->inj_last) instead of an injection launch.
 void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *srcadj)
 {
   const fdmod_params &p = c->p;
@@ -294,7 +300,14 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *src
   AdjArgs a = adj_args(c, false);
   const bool rec = it % c->mt == 0;
   const size_t so = (size_t)(it / c->mt) * c->nr;
-  if (rec && ds.p) launch_adj_inject(s, c->nr, c->g.ldz, c->d_gx, c->d_gz, ds.p + so, ds.p + so, a.mxx, a.mzz, 0.5f);
+  // Receivers on one row of consecutive columns: the two sweeps add the pressure
+  // residuals where they read mu_sigma.  Not with a free surface (FS^T runs between the
+  // injection and the stress adjoint) or the source adjoint (it reads the injected
+  // mu_sigma before the sweeps).
+  const bool fuse_inj = snap_slot >= 0 && rec && ds.p && c->rline && !p.freesurface && !srcadj;
+  c->inj_last = fuse_inj ? ds.p + so : nullptr;
+  if (rec && ds.p && !fuse_inj)
+    launch_adj_inject(s, c->nr, c->g.ldz, c->d_gx, c->d_gz, ds.p + so, ds.p + so, a.mxx, a.mzz, 0.5f);
   if (srcadj && c->ns > 0 && c->stype == 1) {
     const bool tensor = p.form == FDMOD_FORM_TENSOR;  // TR:888-891: the source also feeds tau
     launch_adj_source(s, source_args(c, it), a.mxx, a.mzz, tensor ? a.mtxx : nullptr, tensor ? a.mtzz : nullptr,
@@ -303,6 +316,12 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *src
   if (p.freesurface) launch_adj_fs(s, a);
   AdjArgs as = a;  // the stress adjoint's strips output kTZS rows each
   as.g.tilesZ = (c->g.izEnd - c->g.izStart + kTZS - 1) / kTZS;
+  if (fuse_inj) {
+    as.inj = ds.p + so;
+    as.inj_x0 = c->rline_x0;
+    as.inj_z = c->rline_z;
+    as.inj_n = c->nr;
+  }
   launch_adj_ssweep(s, as);
   if (rec && ds.vx)
     launch_adj_inject(s, c->nr, c->g.ldz, c->d_gx, c->d_gz, ds.vx + so, ds.vz + so, a.mvx, a.mvz, 1.0f);
@@ -318,6 +337,12 @@ void adj_step(fdmod_ctx *c, int k, const DataSrc &ds, int snap_slot, double *src
   av.oxz = mus_field(c, nxt, 2);
   if (snap_slot >= 0)
     for (int q = 3; q < n_snap(p.form); q++) av.snap[q] = fld(c, c->d_snap, (size_t)snap_slot * 6 + q);
+  if (fuse_inj) {
+    av.inj = as.inj;
+    av.inj_x0 = as.inj_x0;
+    av.inj_z = as.inj_z;
+    av.inj_n = as.inj_n;
+  }
   launch_adj_vsweep(s, av);
   c->mus_last = c->mus_cur;
   c->mus_cur = nxt;

@@ -152,6 +152,11 @@ struct AdjArgs {
   float *oxx, *ozz, *oxz;              // V-adjoint: where the updated mu_sigma goes (mxx... in place)
   float *snap[6];                      // V-adjoint: imaging snapshots of computed fields, [3..] (or null)
   double *G[8];                        // imaging sums (per-form meaning, kernels_adjoint.hip)
+  // Pressure residuals of this step for receivers at padded cells (inj_x0 + i, inj_z),
+  // i < inj_n (null: none).  Both adjoint sweeps then add 0.5 inj[i] to mu_sxx and mu_szz
+  // where they read them (the imaging does the same with the step's slot).
+  const float *inj;
+  int inj_x0, inj_z, inj_n;
 };
 
 // One imaging launch over nk <= kImgK steps: the V-adjoint's snapshots of each step
@@ -166,6 +171,10 @@ struct ImgBlock {
   const _Float16 *h16[kImgK][8];
   const float *hsc[kImgK];
   size_t hsc_stride;
+  // steps whose pressure residuals the adjoint sweeps added themselves: the residuals
+  // (receivers at padded cells (inj_x0 + i, inj_z), i < inj_n), else null
+  const float *inj[kImgK];
+  int inj_x0, inj_z, inj_n;
 };
 
 // FP16 packing of nf padded fields: dst = src * 2^(14-E) per (column, 64-row block),

@@ -55,6 +55,7 @@ __device__ __forceinline__ bool region_undamped(const AdjArgs &a, int z_lo, int 
 constexpr int kRowMax = 14;
 struct MRow {
   float v[kRowMax];
+  float r;  // fused injection: the pressure residual of the row's column
 };
 
 template <int FORM>
@@ -126,18 +127,31 @@ struct VAdjOps {
 // passes the row clamped into the grid: rows past the grid feed no output row.  No
 // branch surrounds the loads, so that the look-ahead stays in flight (a load under a
 // branch makes the compiler wait for every load before the next use of an older one).
-template <int FORM>
+template <int FORM, bool INJ>
 __device__ __forceinline__ void load_mrow_lane(MRow &r, const AdjArgs &a, int gx, int rowc, const Grid &g, bool und)
 {
   load_mrow<FORM>(r, a, (size_t)gx * g.ldz + rowc, und);
+  r.r = INJ ? a.inj[min(max(gx - a.inj_x0, 0), a.inj_n - 1)] : 0.0f;  // one address per wave
 }
 
-template <int FORM>
-__device__ __forceinline__ void store_mrow(float (*ring)[kRing][TZ], int slot, const MRow &r, int lane)
+// M of staged column col into the ring.  INJ: lane ilane holds the receiver row, whose
+// mu_sxx and mu_szz take 0.5 r first (the transpose of the pressure receiver, as the
+// separate injection adds it).
+template <int FORM, bool INJ>
+__device__ __forceinline__ void store_mrow(float (*ring)[kRing][TZ], int slot, const MRow &r, int lane, int col,
+                                           int ilane, const AdjArgs &a)
 {
   constexpr int NM = m_fields<FORM>();
   float m[4];
-  m_of<FORM>(r, m);
+  if (INJ && lane == ilane && (unsigned)(col - a.inj_x0) < (unsigned)a.inj_n) {
+    MRow t = r;
+    const float d = 0.5f * r.r;
+    t.v[0] = r.v[0] + d;
+    t.v[1] = r.v[1] + d;
+    m_of<FORM>(t, m);
+  } else {
+    m_of<FORM>(r, m);
+  }
 #pragma unroll
   for (int f = 0; f < NM; f++) ring[f][slot][lane] = m[f];
 }
@@ -145,7 +159,7 @@ __device__ __forceinline__ void store_mrow(float (*ring)[kRing][TZ], int slot, c
 // The strip of block row btz outputs rows z0 .. z0+54 (lanes 4..58) and stages rows
 // z0-4 .. z0+59, one per lane: each staged value is loaded and turned into M once,
 // where 64-row outputs need a second, mostly idle load of the 9 halo rows per field.
-template <int FORM, int PD>
+template <int FORM, int PD, bool INJ>
 __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
 {
   constexpr int NM = m_fields<FORM>();
@@ -161,6 +175,7 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   const int izc = min(max(iz, 0), g.Nz - 1);  // in-grid row for the loads
   const int xe = xa + nc - 1;
   const bool zok = lane >= 4 && lane < 4 + kTZS && iz < g.izEnd;
+  const int ilane = INJ ? a.inj_z - rs : -1;  // the lane staging the receiver row (may be none)
   // staged rows rs .. rs+63 of columns xa-4 .. xa+nc+4
   const bool und = region_undamped(a, rs, rs + TZ, xa - 4, xa + nc + 5);
   // prologue in groups of three columns (a staged row is up to 14 floats)
@@ -168,9 +183,9 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   for (int m0 = 0; m0 < 9; m0 += 3) {
     MRow c[3];
 #pragma unroll
-    for (int m = 0; m < 3; m++) load_mrow_lane<FORM>(c[m], a, xa - 4 + m0 + m, izc, g, und);
+    for (int m = 0; m < 3; m++) load_mrow_lane<FORM, INJ>(c[m], a, xa - 4 + m0 + m, izc, g, und);
 #pragma unroll
-    for (int m = 0; m < 3; m++) store_mrow<FORM>(ring, m0 + m, c[m], lane);
+    for (int m = 0; m < 3; m++) store_mrow<FORM, INJ>(ring, m0 + m, c[m], lane, xa - 4 + m0 + m, ilane, a);
   }
   MRow colq[PD];
   VAdjOps opq[PD];
@@ -183,7 +198,7 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
   // look-ahead loads from clamped columns (values past the segment are not used)
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    load_mrow_lane<FORM>(colq[i], a, min(xa + 5 + i, xe + 5), izc, g, und);
+    load_mrow_lane<FORM, INJ>(colq[i], a, min(xa + 5 + i, xe + 5), izc, g, und);
     load_ops(opq[i], min(xa + i, xe));
   }
   const float invdx = g.invdx, invdz = g.invdz;
@@ -195,9 +210,9 @@ __device__ __forceinline__ void adj_s_tile(const AdjArgs &a, int btz, int btx)
       const int c = xa + k;
       MRow cn;
       VAdjOps on;
-      load_mrow_lane<FORM>(cn, a, min(c + 5 + PD, xe + 5), izc, g, und);
+      load_mrow_lane<FORM, INJ>(cn, a, min(c + 5 + PD, xe + 5), izc, g, und);
       load_ops(on, min(c + PD, xe));
-      store_mrow<FORM>(ring, (u + 9) % kRing, colq[0], lane);
+      store_mrow<FORM, INJ>(ring, (u + 9) % kRing, colq[0], lane, c + 5, ilane, a);
       __syncthreads();
       if (zok) {
         float d1, d2;
@@ -310,7 +325,7 @@ __device__ __forceinline__ void store_ncol(float (*ring)[kRing][LZ], int slot, c
 // computed where they are used, so that no instruction waits on a look-ahead load in the
 // iteration that issues it.
 struct SAdjOps {
-  float s1, s2, s3, k1, t1, t2, t3, S2, K2, S2T, c2, ir, mvx, mvz;
+  float s1, s2, s3, k1, t1, t2, t3, S2, K2, S2T, c2, ir, mvx, mvz, r;
 };
 
 // mu_sigma(q) <- k1(q) mu_sigma(q) - (stress-gradient adjoints)(q), with k1 = coef1 in U
@@ -319,7 +334,7 @@ struct SAdjOps {
 // a = Q^T mu_sigma + mu_tau.  With a.snap set, the incoming values the imaging of this
 // step needs are copied there (points of U): mu_sigma, then a (tensor) or (a, b)
 // (coordinate).
-template <int FORM, int PD>
+template <int FORM, int PD, bool INJ>
 __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
 {
   constexpr int NN = n_fields<FORM>();
@@ -336,6 +351,7 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
   const int rs = z0 - 5;
   const int xe = xa + nc - 1;
   const int izc = min(iz, g.Nz - 1);  // in-grid row for the per-point loads
+  const bool irow = INJ && iz == a.inj_z;  // the receiver row (fused injection)
   // staged rows rs+lane and rs+64+lane, clamped into the grid for the loads
   const int r0 = rs + lane, r1 = rs + TZ + lane;
   const bool ok0 = r0 >= 0 && r0 < g.Nz, ok1 = lane < kHalo && r1 >= 0 && r1 < g.Nz;
@@ -360,6 +376,7 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
     p.s2 = a.mzz[o];
     p.s3 = a.mxz[o];
     p.k1 = und ? 1.0f : a.coef1[o];
+    if (INJ) p.r = a.inj[min(max(col - a.inj_x0, 0), a.inj_n - 1)];  // one address per wave
     if (FORM == FDMOD_FORM_TENSOR) {
       p.t1 = a.mtxx[o];
       p.t2 = a.mtzz[o];
@@ -404,6 +421,15 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
         const size_t o = (size_t)c * g.ldz + iz;
         const SAdjOps &op = opq[0];
         const float k1 = inU ? op.k1 : 1.0f;  // 1 on the image rows above U
+        // fused injection: the receiver's mu_sxx and mu_szz take 0.5 r, as the stress adjoint
+        // staged them (the slot keeps the values without it; the imaging adds it again)
+        float s1 = op.s1, s2 = op.s2;
+        if (INJ) {  // a select, not a branch: a branch here makes the compiler wait on every load
+          const bool hit = irow && (unsigned)(c - a.inj_x0) < (unsigned)a.inj_n;
+          const float d = 0.5f * op.r;
+          s1 = hit ? op.s1 + d : op.s1;
+          s2 = hit ? op.s2 + d : op.s2;
+        }
         float gxx, gzz, gxz;  // adjoint contributions to subtract
         if constexpr (FORM != FDMOD_FORM_COORD) {
           const float nxdx = 0.5f * (dd1[0] + dd2[0]) * invdx, nxdz = 0.5f * (dd1[0] - dd2[0]) * invdz;
@@ -418,9 +444,9 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
           gzz = 0.5f * (dd1[4] + dd2[4]) * invdx + 0.5f * (dd1[5] - dd2[5]) * invdz;
         }
         if (FORM == FDMOD_FORM_TENSOR && inU) {
-          const float a1 = op.K2 * op.s1 + op.S2 * op.s2 - 0.5f * op.S2T * op.s3 + op.t1;
-          const float a2 = op.S2 * op.s1 + op.K2 * op.s2 + 0.5f * op.S2T * op.s3 + op.t2;
-          const float a3 = op.S2T * op.s1 - op.S2T * op.s2 + (op.K2 - op.S2) * op.s3 + op.t3;
+          const float a1 = op.K2 * s1 + op.S2 * s2 - 0.5f * op.S2T * op.s3 + op.t1;
+          const float a2 = op.S2 * s1 + op.K2 * s2 + 0.5f * op.S2T * op.s3 + op.t2;
+          const float a3 = op.S2T * s1 - op.S2T * s2 + (op.K2 - op.S2) * op.s3 + op.t3;
           a.mtxx[o] = k1 * a1;
           a.mtzz[o] = k1 * a2;
           a.mtxz[o] = k1 * a3;
@@ -433,8 +459,8 @@ __device__ __forceinline__ void adj_v_tile(const AdjArgs &a, int btz, int btx)
             a.snap[5][o] = a3;
           }
         } else {
-          a.oxx[o] = k1 * op.s1 - gxx;
-          a.ozz[o] = k1 * op.s2 - gzz;
+          a.oxx[o] = k1 * s1 - gxx;
+          a.ozz[o] = k1 * s2 - gzz;
           a.oxz[o] = k1 * op.s3 - gxz;
           if (FORM == FDMOD_FORM_COORD && a.snap[3] && inU) {
             const float kr = op.c2 * op.ir;
@@ -540,6 +566,12 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
     double L[NL], H[NH];
 #pragma unroll
     for (int q = 0; q < NL; q++) L[q] = b.lam[j][q][o];
+    if (b.inj[j] && iz == b.inj_z && ix >= b.inj_x0 && ix < b.inj_x0 + b.inj_n) {
+      // a step whose pressure residual the sweeps added themselves (the slot is without it)
+      const float d = 0.5f * b.inj[j][ix - b.inj_x0];
+      L[0] = b.lam[j][0][o] + d;
+      L[1] = b.lam[j][1][o] + d;
+    }
     if (F16) {
 #pragma unroll
       for (int q = 0; q < NH; q++) H[q] = (float)b.h16[j][q][o] * b.hsc[j][q * b.hsc_stride + so];
@@ -604,20 +636,20 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
 #endif
 constexpr int kPipeAdj = FDMOD_PIPE_ADJ;
 
-template <int FORM>
+template <int FORM, bool INJ>
 __global__ __launch_bounds__(64) void k_adj_ssweep(AdjArgs a)
 {
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLV, btz, btx)) return;
-  adj_s_tile<FORM, kPipeAdj>(a, btz, btx);
+  adj_s_tile<FORM, kPipeAdj, INJ>(a, btz, btx);
 }
 
-template <int FORM>
+template <int FORM, bool INJ>
 __global__ __launch_bounds__(64) void k_adj_vsweep(AdjArgs a)
 {
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLS, btz, btx)) return;
-  adj_v_tile<FORM, kPipeAdj>(a, btz, btx);
+  adj_v_tile<FORM, kPipeAdj, INJ>(a, btz, btx);
 }
 
 static dim3 tiles_grid(const Grid &g, int segL)
@@ -626,22 +658,36 @@ static dim3 tiles_grid(const Grid &g, int segL)
   return dim3((n + kXCD - 1) / kXCD * kXCD);
 }
 
-#define FDMOD_BY_FORM(KERNEL, GRID, BLOCK, ...)                                                     \
-  do {                                                                                             \
-    if (a.form == FDMOD_FORM_COORD) hipLaunchKernelGGL(KERNEL<FDMOD_FORM_COORD>, GRID, BLOCK, 0, s, __VA_ARGS__);   \
-    else if (a.form == FDMOD_FORM_TENSOR)                                                          \
-      hipLaunchKernelGGL(KERNEL<FDMOD_FORM_TENSOR>, GRID, BLOCK, 0, s, __VA_ARGS__);              \
-    else hipLaunchKernelGGL(KERNEL<FDMOD_FORM_BOND>, GRID, BLOCK, 0, s, __VA_ARGS__);              \
-  } while (0)
+template <bool INJ>
+static void launch_adj_s(hipStream_t s, dim3 grid, const AdjArgs &a)
+{
+  if (a.form == FDMOD_FORM_COORD) hipLaunchKernelGGL((k_adj_ssweep<FDMOD_FORM_COORD, INJ>), grid, dim3(64), 0, s, a);
+  else if (a.form == FDMOD_FORM_TENSOR)
+    hipLaunchKernelGGL((k_adj_ssweep<FDMOD_FORM_TENSOR, INJ>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((k_adj_ssweep<FDMOD_FORM_BOND, INJ>), grid, dim3(64), 0, s, a);
+}
+
+template <bool INJ>
+static void launch_adj_v(hipStream_t s, dim3 grid, const AdjArgs &a)
+{
+  if (a.form == FDMOD_FORM_COORD) hipLaunchKernelGGL((k_adj_vsweep<FDMOD_FORM_COORD, INJ>), grid, dim3(64), 0, s, a);
+  else if (a.form == FDMOD_FORM_TENSOR)
+    hipLaunchKernelGGL((k_adj_vsweep<FDMOD_FORM_TENSOR, INJ>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((k_adj_vsweep<FDMOD_FORM_BOND, INJ>), grid, dim3(64), 0, s, a);
+}
 
 void launch_adj_ssweep(hipStream_t s, const AdjArgs &a)
 {
-  FDMOD_BY_FORM(k_adj_ssweep, tiles_grid(a.g, a.g.segLV), dim3(64), a);
+  const dim3 grid = tiles_grid(a.g, a.g.segLV);
+  if (a.inj) launch_adj_s<true>(s, grid, a);
+  else launch_adj_s<false>(s, grid, a);
 }
 
 void launch_adj_vsweep(hipStream_t s, const AdjArgs &a)
 {
-  FDMOD_BY_FORM(k_adj_vsweep, tiles_grid(a.g, a.g.segLS), dim3(64), a);
+  const dim3 grid = tiles_grid(a.g, a.g.segLS);
+  if (a.inj) launch_adj_v<true>(s, grid, a);
+  else launch_adj_v<false>(s, grid, a);
 }
 
 void launch_image(hipStream_t s, const AdjArgs &a, const ImgBlock &b)
