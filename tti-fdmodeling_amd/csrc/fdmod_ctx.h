@@ -113,6 +113,7 @@ struct fdmod_batch {
   // shots grp[g] .. grp[g+1]-1, its tile prefix sums start at d_startV/S + grp[g] + g,
   // its tile totals are totV/totS[g].  Groups hold up to group_pts padded points.
   std::vector<int> grp, totV, totS;
+  std::vector<char> grp_tau_nt;  // per group: tensor-form tau stored non-temporally
   size_t group_pts = fdm::kBatchGroupPts;
   fdm::SourceArgs *d_srcV = nullptr, *d_srcS = nullptr;
   int srcBlocksV = 0, srcBlocksS = 0;

@@ -69,8 +69,13 @@ struct SweepArgs {
   const float *sin2, *cos2, *sin2t, *sn, *cs, *snv, *csv;
   const uint8_t *undamped;  // per tile: coef1 == 1 and coef2 == dt everywhere in the tile
   int freesurface;
+  // Tensor form: tau stored non-temporally (grids whose fields overflow the 256 MB
+  // last-level cache; on smaller grids tau is still cached when the next stress sweep
+  // reads it).  Read on the host: it selects the kernel.
+  int tau_nt;
   // History output for the gradient, stored at every updated point as FP32 (estore 1)
-  // or as FP16 with one power-of-two scale per field, column and 64-row strip
+  // or as FP16 with one power-of-two scale per column and 64-row strip for the fields
+  // one sweep stores together (copied into each field's scale array)
   // (estore 2; scales [Nx][tilesZ] in hsc[field]):
   //   Bond        stress sweep (vxdx, vzdz, vzdx + vxdz) -> h[0..2]
   //   tensor      stress sweep (vxdx + vzdz, vxdx - vzdz, vzdx + vxdz) -> h[0..2] and
@@ -214,7 +219,7 @@ int sweep_tiles(const Grid &g, int segL);
 // batched launches over many shots: device arrays of per-shot args, tile prefix sums
 void launch_vsweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total);
 void launch_ssweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total,
-                         const int *step, int off, int nt, int mode);
+                         const int *step, int off, int nt, int mode, bool tau_nt);
 void launch_source_batch(hipStream_t s, const SourceArgs *shots, int nshots, int maxblocks, const int *step, int off,
                          int nt, int mode);
 void launch_receivers_batch(hipStream_t s, const RecvArgs *shots, int nshots, int maxnr, const int *step, int off,

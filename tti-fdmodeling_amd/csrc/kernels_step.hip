@@ -21,9 +21,12 @@
 namespace fdm {
 
 // Store one column of NF history fields f0 .. f0+NF-1 for the gradient (all lanes call
-// it; rows past izEnd pass zeros).  FP16: each field is stored as v * 2^(14-E) with E
-// the exponent of its largest |v| in the column's 64-row strip (|stored| < 2^14), and
-// the strip keeps 2^(E-14) for that field.
+// it; rows past izEnd pass zeros).  FP16: the NF fields are stored as v * 2^(14-E) with
+// E the exponent of the largest |v| of any of them in the column's 64-row strip
+// (|stored| < 2^14), and the strip keeps 2^(E-14) in each field's scale array.  One
+// scale for the fields stored together costs no precision: FP16 keeps 11 significant
+// bits for every value within 2^27 of the largest, and one wave reduction per column
+// instead of NF took 2.6% off the coordinate form's FP16-history gradient.
 template <int ESTORE, int NF>
 __device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, int btz, bool zok, const float (&e)[NF],
                                            int f0)
@@ -33,17 +36,21 @@ __device__ __forceinline__ void store_hist(const SweepArgs &a, size_t o, int c, 
     const bool in = c >= a.hskip[0] && c < a.hskip[1] && iz >= a.hskip[2] && iz < a.hskip[3];
     if (zok && in == (a.hinside != 0))
 #pragma unroll
-      for (int f = 0; f < NF; f++) a.h32[f0 + f][o] = e[f];
+      for (int f = 0; f < NF; f++) st_far(a.h32[f0 + f] + o, e[f]);
     return;
   }
+  float mx = 0.0f;
+#pragma unroll
+  for (int f = 0; f < NF; f++) mx = fmaxf(mx, fabsf(e[f]));
+  const float m = wave_max64(mx);
+  int E = 0;
+  if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
+  E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
+  const float sc = ldexpf(1.0f, 14 - E), inv = ldexpf(1.0f, E - 14);
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    const float m = wave_max64(fabsf(e[f]));
-    int E = 0;
-    if (m > 0.0f) (void)frexpf(m, &E);  // m < 2^E
-    E = max(E, -100);                   // keeps 2^(14-E) and 2^(E-14) normal floats
-    if (zok) a.h16[f0 + f][o] = (_Float16)(e[f] * ldexpf(1.0f, 14 - E));
-    if (threadIdx.x == 0) a.hsc[f0 + f][(size_t)c * a.g.tilesZ + btz] = ldexpf(1.0f, E - 14);
+    if (zok) st_far(a.h16[f0 + f] + o, (_Float16)(e[f] * sc));
+    if (threadIdx.x == 0) a.hsc[f0 + f][(size_t)c * a.g.tilesZ + btz] = inv;
   }
 }
 
@@ -196,7 +203,7 @@ __device__ __forceinline__ void load_sops(SOps &p, const SweepArgs &a, size_t o,
 // their ends, so the second load finds them in L2 (6% fewer HBM fetches on a 1256 x 8002
 // grid).  Column xa+j (forward) or xe+j (reversed) lives in ring slot (j+5) or (4-j)
 // mod kRing.  The ring is the caller's, so that both directions share one LDS array.
-template <int FORM, int PD, int ESTORE, bool REV>
+template <int FORM, int PD, int ESTORE, bool REV, bool TNT = false>
 __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx, float (*ring)[kRing][LZ])
 {
   const Grid &g = a.g;
@@ -323,9 +330,20 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
           const float t1 = k1 * op.s1 + k2 * (op.c11 * exx + op.c13 * ezz);
           const float t2 = k1 * op.s2 + k2 * (op.c13 * exx + op.c33 * ezz);
           const float t3 = k1 * op.s3 + k2 * op.c55 * exz;
-          st_out(a.txx + o, t1);
-          st_out(a.tzz + o, t2);
-          st_out(a.txz + o, t3);
+          // tau is next read by the next stress sweep, after the velocity sweep has streamed
+          // its fields through the caches: on large grids (TNT) non-temporal stores leave the
+          // caches to sigma, which the velocity sweep reads next (tensor stress sweep 133.1
+          // -> 129.5 us, 45.9 -> 47.4 Gpts/s on the 1126 x 7872 grid).  A compile-time choice:
+          // with a run-time branch the compiler merges the two stores and drops the hint.
+          if constexpr (TNT) {
+            st_far(a.txx + o, t1);
+            st_far(a.tzz + o, t2);
+            st_far(a.txz + o, t3);
+          } else {
+            st_out(a.txx + o, t1);
+            st_out(a.tzz + o, t2);
+            st_out(a.txz + o, t3);
+          }
           // tau stored as (txx + tzz, txx - tzz, txz), for the same reason
           hv[3] = a.rev ? op.s1 + op.s2 : t1 + t2;
           hv[4] = a.rev ? op.s1 - op.s2 : t1 - t2;
@@ -456,14 +474,14 @@ __global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
   vsweep_tile<COORD, PD, ESTORE>(a, btz, btx);
 }
 
-template <int FORM, int PD, int ESTORE = 0>
+template <int FORM, int PD, int ESTORE = 0, bool TNT = false>
 __global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
 {
   __shared__ float ring[2][kRing][LZ];
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLS, btz, btx)) return;
-  if (btx & 1) ssweep_tile<FORM, PD, ESTORE, true>(a, btz, btx, ring);
-  else ssweep_tile<FORM, PD, ESTORE, false>(a, btz, btx, ring);
+  if (btx & 1) ssweep_tile<FORM, PD, ESTORE, true, TNT>(a, btz, btx, ring);
+  else ssweep_tile<FORM, PD, ESTORE, false, TNT>(a, btz, btx, ring);
 }
 
 // Time index of loop iteration k = *step + off (BT:651): it = k or nt-1-k.
@@ -508,7 +526,7 @@ __global__ __launch_bounds__(64) void k_vsweep_batch(const SweepArgs *__restrict
 // Shots with a fused stress source or pressure receivers (as in the single-shot loop)
 // take the iteration from the device step counter: the source sample and, when
 // it % mt == 0, the receivers' output row.
-template <int FORM, int PD>
+template <int FORM, int PD, bool TNT = false>
 __global__ __launch_bounds__(64) void k_ssweep_batch(const SweepArgs *__restrict__ shots, const int *__restrict__ start,
                                                      int nshots, const int *step, int off, int nt, int mode)
 {
@@ -525,8 +543,8 @@ __global__ __launch_bounds__(64) void k_ssweep_batch(const SweepArgs *__restrict
     }
   }
   const int btx = lt / a.g.tilesZ;
-  if (btx & 1) ssweep_tile<FORM, PD, 0, true>(a, lt - btx * a.g.tilesZ, btx, ring);
-  else ssweep_tile<FORM, PD, 0, false>(a, lt - btx * a.g.tilesZ, btx, ring);
+  if (btx & 1) ssweep_tile<FORM, PD, 0, true, TNT>(a, lt - btx * a.g.tilesZ, btx, ring);
+  else ssweep_tile<FORM, PD, 0, false, TNT>(a, lt - btx * a.g.tilesZ, btx, ring);
 }
 
 static dim3 sweep_grid(const Grid &g, int segL)
@@ -546,6 +564,12 @@ static void launch_v(hipStream_t s, dim3 grid, const SweepArgs &a)
 template <int FORM, int ESTORE>
 static void launch_s(hipStream_t s, dim3 grid, const SweepArgs &a)
 {
+  if constexpr (FORM == FDMOD_FORM_TENSOR) {
+    if (a.tau_nt) {
+      hipLaunchKernelGGL((k_ssweep<FORM, pipe_s<FORM>(), ESTORE, true>), grid, dim3(64), 0, s, a);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_ssweep<FORM, pipe_s<FORM>(), ESTORE>), grid, dim3(64), 0, s, a);
 }
 
@@ -584,12 +608,15 @@ void launch_vsweep_batch(hipStream_t s, int form, const SweepArgs *shots, const 
 }
 
 void launch_ssweep_batch(hipStream_t s, int form, const SweepArgs *shots, const int *start, int nshots, int total,
-                         const int *step, int off, int nt, int mode)
+                         const int *step, int off, int nt, int mode, bool tau_nt)
 {
   dim3 grid((total + kXCD - 1) / kXCD * kXCD);
   if (form == FDMOD_FORM_COORD)
     hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_COORD, pipe_s<FDMOD_FORM_COORD>()>), grid, dim3(64), 0, s, shots, start,
                        nshots, step, off, nt, mode);
+  else if (form == FDMOD_FORM_TENSOR && tau_nt)
+    hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_TENSOR, pipe_s<FDMOD_FORM_TENSOR>(), true>), grid, dim3(64), 0, s,
+                       shots, start, nshots, step, off, nt, mode);
   else if (form == FDMOD_FORM_TENSOR)
     hipLaunchKernelGGL((k_ssweep_batch<FDMOD_FORM_TENSOR, pipe_s<FDMOD_FORM_TENSOR>()>), grid, dim3(64), 0, s, shots,
                        start, nshots, step, off, nt, mode);

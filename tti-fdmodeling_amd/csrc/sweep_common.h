@@ -20,6 +20,10 @@ constexpr int kHalo = LZ - TZ;       // rows loaded by the first kHalo lanes a s
 __device__ __forceinline__ float ldnt(const float *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ float ldst(const float *p) { return *p; }
 __device__ __forceinline__ void st_out(float *p, float v) { *p = v; }
+// Outputs whose next reader comes after other sweeps have streamed through the caches
+// (the tensor form's tau, the gradient's stored history): non-temporal stores.
+template <typename T>
+__device__ __forceinline__ void st_far(T *p, T v) { __builtin_nontemporal_store(v, p); }
 
 // Maximum of v over the 64 lanes of the wave, the same in every lane.  DPP moves:
 // within each row of 16 lanes a butterfly (quad_perm [1,0,3,2] and [2,3,0,1], then
