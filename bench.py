@@ -531,11 +531,14 @@ def main():
 
     from fdmod_hip import Modeling
 
-    # The per-sweep timing runs first: its 200 loop iterations also bring the GPU out of
-    # idle before the warm-up and timed steps (a 20-step run straight after the setup ran
-    # at 55.5 Gpts/s, after 200 steps of warm-up at 58.7).
+    # The per-sweep timing runs first: its loop iterations also bring the GPU out of idle
+    # before the warm-up and timed steps (a 20-step run straight after the setup ran at
+    # 55.5 Gpts/s, after 200 steps of warm-up at 58.7).  Its first 100 iterations are
+    # discarded: started on an idle GPU they read the velocity sweep 10% slow.
     g = make(Modeling, args.form, cfg, m, src)
     log("bench: per-sweep HIP-event timing")
+    g.time_sweeps(100)
+    g.reset()
     v_ms, s_ms = g.time_sweeps(200)
     g.reset()
     log(f"bench: timing {args.form}, {args.steps} steps after {args.warmup} warm-up steps")

@@ -538,7 +538,7 @@ void launch_pack16(hipStream_t s, const Pack16 &p, bool pack)
 //           snapshots mu_sigma and a = Q^T mu_sigma + mu_tau, the strain and tau_{k+1}.
 //   coord   G = c11, c13, c33, c55, sin, cos at stress points (CR:821-843) and sinv,
 //           cosv at velocity points (CR:663-670).
-template <int FORM, bool F16>
+template <int FORM, bool F16, bool INJ>
 __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
 {
   const Grid &g = a.g;
@@ -566,14 +566,23 @@ __global__ __launch_bounds__(256) void k_image(AdjArgs a, ImgBlock b)
     constexpr int j = decltype(jc)::value;
     if (j >= b.nk) return;
     double L[NL], H[NH];
+    float l[NL];
 #pragma unroll
-    for (int q = 0; q < NL; q++) L[q] = b.lam[j][q][o];
-    if (b.inj[j] && iz == b.inj_z && ix >= b.inj_x0 && ix < b.inj_x0 + b.inj_n) {
-      // a step whose pressure residual the sweeps added themselves (the slot is without it)
-      const float d = 0.5f * b.inj[j][ix - b.inj_x0];
-      L[0] = b.lam[j][0][o] + d;
-      L[1] = b.lam[j][1][o] + d;
+    for (int q = 0; q < NL; q++) l[q] = b.lam[j][q][o];
+    if (INJ) {
+      // steps whose pressure residuals the sweeps added themselves (the slot is without
+      // them): selects and an unconditional load, no branch, so that the unrolled steps'
+      // loads stay in flight together
+      const bool on = b.inj[j] != nullptr;
+      const float *ip = on ? b.inj[j] : b.lam[j][0];
+      const float r = ip[on ? min(max(ix - b.inj_x0, 0), b.inj_n - 1) : 0];
+      const bool hit = on && iz == b.inj_z && (unsigned)(ix - b.inj_x0) < (unsigned)b.inj_n;
+      const float d = 0.5f * r;
+      l[0] = hit ? l[0] + d : l[0];
+      l[1] = hit ? l[1] + d : l[1];
     }
+#pragma unroll
+    for (int q = 0; q < NL; q++) L[q] = l[q];
     if (F16) {
 #pragma unroll
       for (int q = 0; q < NH; q++) H[q] = (float)b.h16[j][q][o] * b.hsc[j][q * b.hsc_stride + so];
@@ -696,10 +705,14 @@ void launch_image(hipStream_t s, const AdjArgs &a, const ImgBlock &b)
 {
   dim3 grid((a.g.izEnd - a.g.izStart + 255) / 256, a.g.ixEnd - a.g.ixStart);
   const bool f16 = b.h16[0][0] != nullptr;
+  bool inj = false;
+  for (int j = 0; j < b.nk; j++) inj = inj || b.inj[j] != nullptr;
 #define FDMOD_IMG(F)                                                                                \
   do {                                                                                             \
-    if (f16) hipLaunchKernelGGL((k_image<F, true>), grid, dim3(256), 0, s, a, b);                  \
-    else hipLaunchKernelGGL((k_image<F, false>), grid, dim3(256), 0, s, a, b);                     \
+    if (f16 && inj) hipLaunchKernelGGL((k_image<F, true, true>), grid, dim3(256), 0, s, a, b);      \
+    else if (f16) hipLaunchKernelGGL((k_image<F, true, false>), grid, dim3(256), 0, s, a, b);      \
+    else if (inj) hipLaunchKernelGGL((k_image<F, false, true>), grid, dim3(256), 0, s, a, b);      \
+    else hipLaunchKernelGGL((k_image<F, false, false>), grid, dim3(256), 0, s, a, b);              \
   } while (0)
   if (a.form == FDMOD_FORM_COORD) FDMOD_IMG(FDMOD_FORM_COORD);
   else if (a.form == FDMOD_FORM_TENSOR) FDMOD_IMG(FDMOD_FORM_TENSOR);
