@@ -506,6 +506,14 @@ SweepArgs sweep_args(fdmod_ctx *c)
   // against twice the last-level cache: 1256 x 8002 -> non-temporal; a 1126 x 2000 crop
   // (1246 x 2120) -> cached (its gradient ran 1.3% slower with non-temporal tau)
   a.tau_nt = 20.0 * 4.0 * c->g.Nx * c->g.ldz > 512.0 * (1 << 20);
+  // The same rule for the per-point operands, counting the arrays each form streams per
+  // step (Bond 14, coordinate 16, tensor 20).  Measured on MI355X: the full 1126 x 7872
+  // grid (1256 x 8002) streams 573-819 MB and runs faster with non-temporal operand
+  // loads (cached, the stress sweep took 9 us more per launch); 1126-deep crops 1500-4000
+  // columns wide stream 120-420 MB and ran their forward steps 5-7% faster with cached
+  // loads, the 1126 x 2000 gradients 3-6% faster (Bond FP16 history 0.54 -> 0.525 s).
+  const int arrays = c->p.form == FDMOD_FORM_TENSOR ? 20 : c->p.form == FDMOD_FORM_COORD ? 16 : 14;
+  a.op_nt = arrays * 4.0 * c->g.Nx * c->g.ldz > 512.0 * (1 << 20);
   return a;
 }
 

@@ -73,6 +73,10 @@ struct SweepArgs {
   // last-level cache; on smaller grids tau is still cached when the next stress sweep
   // reads it).  Read on the host: it selects the kernel.
   int tau_nt;
+  // Per-point operands (old state, parameters) loaded non-temporally: grids whose arrays
+  // one step streams overflow twice the 256 MB last-level cache.  On smaller grids (the
+  // gradient's crops) they load cached, and part of them is still there the next step.
+  int op_nt;
   // History output for the gradient, stored at every updated point as FP32 (estore 1)
   // or as FP16 with one power-of-two scale per column and 64-row strip for the fields
   // one sweep stores together (copied into each field's scale array)

@@ -59,17 +59,22 @@ struct VOps {
   float vx, vz, ir, k1, k2, cv, sv;
 };
 
-template <bool COORD>
+// OPNT: the per-point operands load non-temporally (grids whose streamed arrays overflow
+// the last-level cache); on smaller grids they load cached.
+template <bool OPNT>
+__device__ __forceinline__ float ldop(const float *p) { return OPNT ? ldnt(p) : ldst(p); }
+
+template <bool COORD, bool OPNT>
 __device__ __forceinline__ void load_vops(VOps &p, const SweepArgs &a, size_t o, bool und, float dt)
 {
-  p.vx = ldnt(a.vx + o);
-  p.vz = ldnt(a.vz + o);
-  p.ir = ldnt(a.invrho + o);
-  p.k1 = und ? 1.0f : ldnt(a.coef1 + o);
-  p.k2 = und ? dt : ldnt(a.coef2 + o);
+  p.vx = ldop<OPNT>(a.vx + o);
+  p.vz = ldop<OPNT>(a.vz + o);
+  p.ir = ldop<OPNT>(a.invrho + o);
+  p.k1 = und ? 1.0f : ldop<OPNT>(a.coef1 + o);
+  p.k2 = und ? dt : ldop<OPNT>(a.coef2 + o);
   if (COORD) {
-    p.cv = ldnt(a.csv + o);
-    p.sv = ldnt(a.snv + o);
+    p.cv = ldop<OPNT>(a.csv + o);
+    p.sv = ldop<OPNT>(a.snv + o);
   } else {
     p.cv = p.sv = 0.0f;
   }
@@ -79,7 +84,7 @@ __device__ __forceinline__ void load_vops(VOps &p, const SweepArgs &a, size_t o,
 // Coordinate (COORD): CR:614-671, with the reference's invdx on d(sxx)/dz.
 // The loads of the staged column and of the per-point operands run PD iterations
 // ahead of their use.
-template <bool COORD, int PD, int ESTORE = 0>
+template <bool COORD, int PD, int ESTORE = 0, bool OPNT = true>
 __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx)
 {
   __shared__ float ring[3][kRing][LZ];
@@ -109,7 +114,7 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
 #pragma unroll
   for (int i = 0; i < PD; i++) {
     if (i <= nc - 1) load_col<3>(colq[i], fs, xa + 5 + i, rs, lane, g);
-    if (zok && i < nc) load_vops<COORD>(opq[i], a, (size_t)(xa + i) * g.ldz + iz, und, g.dt);
+    if (zok && i < nc) load_vops<COORD, OPNT>(opq[i], a, (size_t)(xa + i) * g.ldz + iz, und, g.dt);
   }
   const float invdx = g.invdx, invdz = g.invdz;
   for (int k0 = 0; k0 < nc; k0 += kRing) {
@@ -123,7 +128,7 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
       // look-ahead loads with no branch around them (clamped column and row; the values
       // past the segment or the grid are not used)
       load_col_nb<3>(cn, fs, min(c + 5 + PD, xe + 5), rs, lane, g);
-      load_vops<COORD>(on, a, (size_t)min(c + PD, xe) * g.ldz + izc, und, g.dt);
+      load_vops<COORD, OPNT>(on, a, (size_t)min(c + PD, xe) * g.ldz + izc, und, g.dt);
       store_col<3>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
       float hv[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // coordinate-form history (ESTORE)
@@ -177,20 +182,20 @@ struct SOps {
   float k1, k2;
 };
 
-template <int FORM>
+template <int FORM, bool OPNT>
 __device__ __forceinline__ void load_sops(SOps &p, const SweepArgs &a, size_t o, bool und, float dt)
 {
   if (FORM == FDMOD_FORM_TENSOR) {
-    p.s1 = ldnt(a.txx + o); p.s2 = ldnt(a.tzz + o); p.s3 = ldnt(a.txz + o);
+    p.s1 = ldop<OPNT>(a.txx + o); p.s2 = ldop<OPNT>(a.tzz + o); p.s3 = ldop<OPNT>(a.txz + o);
   } else {
-    p.s1 = ldnt(a.sxx + o); p.s2 = ldnt(a.szz + o); p.s3 = ldnt(a.sxz + o);
+    p.s1 = ldop<OPNT>(a.sxx + o); p.s2 = ldop<OPNT>(a.szz + o); p.s3 = ldop<OPNT>(a.sxz + o);
   }
-  p.c11 = ldnt(a.c11 + o); p.c13 = ldnt(a.c13 + o); p.c33 = ldnt(a.c33 + o); p.c55 = ldnt(a.c55 + o);
-  if (FORM == FDMOD_FORM_BOND) { p.a = ldnt(a.c15 + o); p.b = ldnt(a.c35 + o); p.c = 0.0f; }
-  else if (FORM == FDMOD_FORM_COORD) { p.a = ldnt(a.sn + o); p.b = ldnt(a.cs + o); p.c = 0.0f; }
-  else { p.a = ldnt(a.sin2 + o); p.b = ldnt(a.cos2 + o); p.c = ldnt(a.sin2t + o); }
-  p.k1 = und ? 1.0f : ldnt(a.coef1 + o);
-  p.k2 = und ? dt : ldnt(a.coef2 + o);
+  p.c11 = ldop<OPNT>(a.c11 + o); p.c13 = ldop<OPNT>(a.c13 + o); p.c33 = ldop<OPNT>(a.c33 + o); p.c55 = ldop<OPNT>(a.c55 + o);
+  if (FORM == FDMOD_FORM_BOND) { p.a = ldop<OPNT>(a.c15 + o); p.b = ldop<OPNT>(a.c35 + o); p.c = 0.0f; }
+  else if (FORM == FDMOD_FORM_COORD) { p.a = ldop<OPNT>(a.sn + o); p.b = ldop<OPNT>(a.cs + o); p.c = 0.0f; }
+  else { p.a = ldop<OPNT>(a.sin2 + o); p.b = ldop<OPNT>(a.cos2 + o); p.c = ldop<OPNT>(a.sin2t + o); }
+  p.k1 = und ? 1.0f : ldop<OPNT>(a.coef1 + o);
+  p.k2 = und ? dt : ldop<OPNT>(a.coef2 + o);
 }
 
 // Stress sweep (A2 + A4/A5/A6), with the free-surface stress imaging (BT:875-891)
@@ -203,7 +208,7 @@ __device__ __forceinline__ void load_sops(SOps &p, const SweepArgs &a, size_t o,
 // their ends, so the second load finds them in L2 (6% fewer HBM fetches on a 1256 x 8002
 // grid).  Column xa+j (forward) or xe+j (reversed) lives in ring slot (j+5) or (4-j)
 // mod kRing.  The ring is the caller's, so that both directions share one LDS array.
-template <int FORM, int PD, int ESTORE, bool REV, bool TNT = false>
+template <int FORM, int PD, int ESTORE, bool REV, bool TNT = false, bool OPNT = true>
 __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx, float (*ring)[kRing][LZ])
 {
   const Grid &g = a.g;
@@ -258,7 +263,7 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
 #pragma unroll
   for (int i = 0; i < PD; i++) {
     if (i <= nc - 1) load_col<2>(colq[i], fv, REV ? xe - 5 - i : xa + 4 + i, rs, lane, g);
-    if (zok && i < nc) load_sops<FORM>(opq[i], a, (size_t)(REV ? xe - i : xa + i) * g.ldz + iz, und, g.dt);
+    if (zok && i < nc) load_sops<FORM, OPNT>(opq[i], a, (size_t)(REV ? xe - i : xa + i) * g.ldz + iz, und, g.dt);
   }
   const float invdx = g.invdx, invdz = g.invdz;
   for (int k0 = 0; k0 < nc; k0 += kRing) {
@@ -272,7 +277,7 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
       // look-ahead loads with no branch around them (clamped column and row; the values
       // past the segment or the grid are not used)
       load_col_nb<2>(cn, fv, REV ? max(c - 5 - PD, xa - 5) : min(c + 4 + PD, xe + 4), rs, lane, g);
-      load_sops<FORM>(on, a, (size_t)(REV ? max(c - PD, xa) : min(c + PD, xe)) * g.ldz + izc, und, g.dt);
+      load_sops<FORM, OPNT>(on, a, (size_t)(REV ? max(c - PD, xa) : min(c + PD, xe)) * g.ldz + izc, und, g.dt);
       store_col<2>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
       auto slot = [](int dx) { return REV ? (u - dx + 4 + kRing) % kRing : (u + dx + 5 + kRing) % kRing; };
@@ -466,22 +471,22 @@ template <int FORM>
 constexpr int pipe_s() { return FDMOD_PIPE_S; }
 
 // Single grid: blocks map to tiles of one shot.
-template <bool COORD, int PD, int ESTORE = 0>
+template <bool COORD, int PD, int ESTORE = 0, bool OPNT = true>
 __global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
 {
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLV, btz, btx)) return;
-  vsweep_tile<COORD, PD, ESTORE>(a, btz, btx);
+  vsweep_tile<COORD, PD, ESTORE, OPNT>(a, btz, btx);
 }
 
-template <int FORM, int PD, int ESTORE = 0, bool TNT = false>
+template <int FORM, int PD, int ESTORE = 0, bool TNT = false, bool OPNT = true>
 __global__ __launch_bounds__(64) void k_ssweep(SweepArgs a)
 {
   __shared__ float ring[2][kRing][LZ];
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLS, btz, btx)) return;
-  if (btx & 1) ssweep_tile<FORM, PD, ESTORE, true, TNT>(a, btz, btx, ring);
-  else ssweep_tile<FORM, PD, ESTORE, false, TNT>(a, btz, btx, ring);
+  if (btx & 1) ssweep_tile<FORM, PD, ESTORE, true, TNT, OPNT>(a, btz, btx, ring);
+  else ssweep_tile<FORM, PD, ESTORE, false, TNT, OPNT>(a, btz, btx, ring);
 }
 
 // Time index of loop iteration k = *step + off (BT:651): it = k or nt-1-k.
@@ -558,19 +563,18 @@ int sweep_tiles(const Grid &g, int segL) { return g.tilesZ * ((g.ixEnd - g.ixSta
 template <bool COORD, int ESTORE>
 static void launch_v(hipStream_t s, dim3 grid, const SweepArgs &a)
 {
-  hipLaunchKernelGGL((k_vsweep<COORD, kPipeV, ESTORE>), grid, dim3(64), 0, s, a);
+  if (a.op_nt) hipLaunchKernelGGL((k_vsweep<COORD, kPipeV, ESTORE, true>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((k_vsweep<COORD, kPipeV, ESTORE, false>), grid, dim3(64), 0, s, a);
 }
 
+// The tensor form's tau_nt and op_nt hold on the same grids (both count its 20 streamed
+// arrays), so its sweeps come in two variants, as do the other forms'.
 template <int FORM, int ESTORE>
 static void launch_s(hipStream_t s, dim3 grid, const SweepArgs &a)
 {
-  if constexpr (FORM == FDMOD_FORM_TENSOR) {
-    if (a.tau_nt) {
-      hipLaunchKernelGGL((k_ssweep<FORM, pipe_s<FORM>(), ESTORE, true>), grid, dim3(64), 0, s, a);
-      return;
-    }
-  }
-  hipLaunchKernelGGL((k_ssweep<FORM, pipe_s<FORM>(), ESTORE>), grid, dim3(64), 0, s, a);
+  constexpr bool T = FORM == FDMOD_FORM_TENSOR;
+  if (a.op_nt) hipLaunchKernelGGL((k_ssweep<FORM, pipe_s<FORM>(), ESTORE, T, true>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((k_ssweep<FORM, pipe_s<FORM>(), ESTORE, false, false>), grid, dim3(64), 0, s, a);
 }
 
 void launch_vsweep(hipStream_t s, int form, const SweepArgs &a)
