@@ -249,8 +249,9 @@ void launch_adj_source(hipStream_t s, const SourceArgs &a, const float *ma, cons
 void launch_residual(hipStream_t s, const float *d, float *r, size_t n, double *partial, int nb, double *out);
 void launch_dot(hipStream_t s, const float *a, const float *bf, const double *bd, size_t n, double *partial, int nb,
                 double *out);
+// Chain rule into grad [4][nx][nz]; overwrites G[0..3] (the imaging sums are spent).
 void launch_chain(hipStream_t s, const Grid &g, int form, int nx, int nz, int fs_row, const DevArrays &d,
-                  const double *G, size_t gstride, double theta_scale, double *grad);
+                  double *G, size_t gstride, double theta_scale, double *grad);
 void launch_fold_trig(hipStream_t s, const Grid &g, double *G, size_t gstride);
 void launch_rescale_frozen(hipStream_t s, const Grid &g, float *vp, float *vs, float *rho, const float *Beta);
 void launch_pack16(hipStream_t s, const Pack16 &p, bool pack);
