@@ -144,9 +144,10 @@ def test_gradient_storage_modes(form):
         assert J == ref[0]  # the forward itself is FP32 in every mode
         errs = [rel_l2(grad[i], gr[i]) for i in range(4)]
         print(form, storage, "history" if hist else "checkpoints", ["%.2e" % e for e in errs], flush=True)
-        # FP16 checkpoints restart each recomputed segment from rounded states; here they
-        # reach 1.4e-3 (tensor form, theta), the other modes stay below 4e-4
-        assert max(errs) <= (2e-3 if storage == "f16all" else 1e-3), errs
+        # FP16 checkpoints restart each recomputed segment from rounded states (normal
+        # stresses stored as sum and difference): here at most 7.8e-4 (tensor form, theta),
+        # the other modes below 4e-4
+        assert max(errs) <= 1e-3, errs
 
 
 # Pressure receivers on one row at consecutive columns (the gradient workloads' geometry):

@@ -139,6 +139,12 @@ int store_io(fdmod_ctx *c, FieldStore &st, size_t slot, int nf, float *const *fi
     pk.src[f] = fields[f];
     pk.dst[f] = fld(c, (_Float16 *)st.data, slot * nf + f);
     pk.inv[f] = st.scale + (slot * nf + f) * ns;
+    pk.mate[f] = -1;
+  }
+  // normal stresses as sum and difference (kStateIds: sxx, szz = 2, 3; txx, tzz = 5, 6)
+  for (int a = 2; a + 1 < nf && a <= 5; a += 3) {
+    pk.mate[a] = a + 1;
+    pk.mate[a + 1] = a;
   }
   launch_pack16(c->stream, pk, save);
   return FDMOD_OK;

@@ -187,12 +187,15 @@ struct ImgBlock {
 };
 
 // FP16 packing of nf padded fields: dst = src * 2^(14-E) per (column, 64-row block),
-// inv = 2^(E-14) per block ([nx][ldz/64]); unpacking writes src back.
+// inv = 2^(E-14) per block ([nx][ldz/64]); unpacking writes src back.  A pair of fields
+// (a, b) with mate[a] = b, mate[b] = a and a < b is packed as a + b in slot a and a - b in
+// slot b, and unpacked as (S + D) / 2 and (S - D) / 2 (mate -1: the field alone).
 struct Pack16 {
   int nx, ldz, nf;  // nf <= 8
   float *src[8];
   _Float16 *dst[8];
   float *inv[8];
+  int mate[8];
 };
 
 // Compact copy of a set of up to 4 rectangles of grid points (rows fastest inside
