@@ -279,7 +279,18 @@ def test_config5_batched_fp16_gradients_at_size():
     nt = GRAD["nt"]
     full, true = _large(), _large_true()
     nx_full = full["vp"].shape[0]
-    crops = _crops(3, 1500, 3000, 5, nx_full)
+    # uneven crops, each with its source above one of the anomalies, so every shot's record sees one
+    d = np.abs(true["vp"] - full["vp"]).max(axis=1)
+    centres = []
+    for cidx in np.argsort(-d):
+        if all(abs(int(cidx) - c0) > 800 for c0 in centres):
+            centres.append(int(cidx))
+        if len(centres) == 3:
+            break
+    crops = []
+    for w, sx in zip((1500, 3000, 2200), centres):
+        x0 = int(min(max(sx - w // 2, 0), nx_full - w))
+        crops.append((x0, w, sx - x0))
     print("C5: crops", [w for _, w, _ in crops], "nt =", nt, flush=True)
     c = GRAD
     kw = dict(ealThick=c["ealThick"], freesurface=c["freesurface"], fpeak=c["fpeak"])
@@ -310,7 +321,8 @@ def test_config5_batched_fp16_gradients_at_size():
         J32, g32 = g.gradient(obs_p=obs[s])
         assert J16 == J32
         errs = [rel_l2(g16[i], g32[i]) for i in range(4)]
-        print("  shot", s, "width", w, "FP16 vs FP32", ["%.2e" % e for e in errs], flush=True)
+        print("  shot", s, "width", w, "misfit", J32, "FP16 vs FP32", ["%.2e" % e for e in errs], flush=True)
+        assert J32 > 0 and np.abs(g32).max() > 0, s
         assert max(errs) <= 1e-3, errs
         ref[:, x0:x0 + w] += g16
         Js += J16
