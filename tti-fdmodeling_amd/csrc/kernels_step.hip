@@ -463,6 +463,13 @@ __device__ __forceinline__ void ssweep_tile(const SweepArgs &a, int btz, int btx
 #define FDMOD_PIPE_S 1
 #endif
 constexpr int kPipeV = FDMOD_PIPE_V;  // prefetch depth (iterations) of the velocity sweep
+// Single-shot velocity sweep of the Bond and tensor forms: three columns ahead.  It loads
+// two operand words per point fewer than the coordinate form's (no velocity-point trig),
+// so it needs more iterations in flight to keep as many bytes in flight.  Measured on
+// the 1256 x 8002 grid (Gpts/s of the loop, depth 2 / 3 / 4): Bond 61.0 / 62.3 / 61.9,
+// tensor 47.3 / 47.9 / 47.6, coordinate 60.8 / 60.2 / 59.6.
+template <bool COORD>
+constexpr int pipe_v() { return COORD ? kPipeV : kPipeV + 1; }
 // Stress sweep: one column ahead for every form (Bond and coordinate 89 VGPRs, tensor 91,
 // five waves per SIMD; the tensor form at depth 2 needs 103, four waves).  Measured on
 // the 1256 x 8002 grid with branch-free look-ahead loads: tensor stress sweep 134.7 us at
@@ -563,8 +570,8 @@ int sweep_tiles(const Grid &g, int segL) { return g.tilesZ * ((g.ixEnd - g.ixSta
 template <bool COORD, int ESTORE>
 static void launch_v(hipStream_t s, dim3 grid, const SweepArgs &a)
 {
-  if (a.op_nt) hipLaunchKernelGGL((k_vsweep<COORD, kPipeV, ESTORE, true>), grid, dim3(64), 0, s, a);
-  else hipLaunchKernelGGL((k_vsweep<COORD, kPipeV, ESTORE, false>), grid, dim3(64), 0, s, a);
+  if (a.op_nt) hipLaunchKernelGGL((k_vsweep<COORD, pipe_v<COORD>(), ESTORE, true>), grid, dim3(64), 0, s, a);
+  else hipLaunchKernelGGL((k_vsweep<COORD, pipe_v<COORD>(), ESTORE, false>), grid, dim3(64), 0, s, a);
 }
 
 // The tensor form's tau_nt and op_nt hold on the same grids (both count its 20 streamed
