@@ -115,6 +115,21 @@ def test_adjoint_and_gradient_dx_ne_dz(form, fs, stype, seis):
     assert max(errs) <= 1e-4, errs
 
 
+@pytest.mark.parametrize("form", FORMS)
+def test_gradient_repeat_runs_bitwise(form):
+    # every gradient mode is deterministic: the FP64 imaging sums and reductions add in a
+    # fixed order, so a second call (new context or the same one) gives the same bits
+    m, obs = _obs(1, 2, form=form)
+    g = _mk(Modeling, m, 1, seis=2, form=form)
+    h = _mk(Modeling, m, 1, seis=2, form=form)
+    for kw in (dict(), dict(storage="f16", history=True), dict(storage="f16all"), dict(history="reconstruct")):
+        J1, g1 = g.gradient(*obs, **kw)
+        J2, g2 = g.gradient(*obs, **kw)
+        J3, g3 = h.gradient(*obs, **kw)
+        assert J1 == J2 == J3, kw
+        assert np.array_equal(g1, g2) and np.array_equal(g1, g3), kw
+
+
 def test_gradient_independent_of_checkpoint_count():
     # the recomputed forward is bit-identical; only the grouping of the FP64 imaging
     # additions (blocks of steps close at segment ends) changes with the count
