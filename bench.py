@@ -240,6 +240,7 @@ def _rel(a, b):
 
 
 GRAD_MODES = {"plain_f32_checkpoints": dict(), "f16_history": dict(storage="f16", history=True),
+              "f16_checkpoints": dict(storage="f16all"),
               "f32_reconstruct": dict(history="reconstruct"),
               "f32_reconstruct_interior": dict(history="reconstruct_interior")}
 
@@ -278,6 +279,8 @@ def gradient_single(full, nt, modes=tuple(GRAD_MODES), forms=("bond", "tensor", 
         ref = res.get("plain_f32_checkpoints")
         if ref is not None and "f16_history" in res:
             fo["f16_vs_f32_rel_l2"] = [round(x, 7) for x in _rel(res["f16_history"], ref)]
+        if ref is not None and "f16_checkpoints" in res:
+            fo["f16_checkpoints_vs_f32_rel_l2"] = [round(x, 7) for x in _rel(res["f16_checkpoints"], ref)]
         if ref is not None and "f32_reconstruct" in res:
             fo["reconstruct_vs_f32_rel_l2"] = [float("%.3g" % x) for x in _rel(res["f32_reconstruct"], ref)]
         if ref is not None and "f32_reconstruct_interior" in res:
@@ -363,9 +366,9 @@ def gradient_batched(full, true_full, nt, nshots, seed=5):
             "batched_vs_plain_rel_l2": [round(x, 7) for x in _rel(gb, gp)]}
 
 
-def cpu_gradient_c4(nt=4000):
+def cpu_gradient_c4(nt=4000, form="bond"):
     """The FP64 oracle gradient on the gradient workload's own geometry (1126 x 2000 crop,
-    nt = 4000, Bond form, checkpointed recomputation, OpenMP on the host cores) and the GPU
+    nt = 4000, checkpointed recomputation, OpenMP on the host cores) and the GPU
     gradients' relative L2 errors against it, per parameter (vp, eps, delta, theta), with
     the observed data shared (modelled by the FP64 oracle) and with each path's own data
     (the GPU's FP32 modelling for the GPU gradients); the FP32 oracle (the reference's
@@ -382,39 +385,39 @@ def cpu_gradient_c4(nt=4000):
     W, x0 = 2000, 2936
     bg = synth.crop(synth.model_large(), x0, W)
     tr = synth.add_anomalies(bg)
-    log("bench: FP64 oracle forward on the true model (1126 x 2000, nt = %d)" % nt)
+    log("bench: FP64 oracle forward on the true model (1126 x 2000, nt = %d, %s form)" % (nt, form))
     t0 = time.perf_counter()
-    ot = _grad_shot(oracle.Oracle, tr, W, W // 2, nt, prec="f64")
+    ot = _grad_shot(oracle.Oracle, tr, W, W // 2, nt, form=form, prec="f64")
     ot.run()
     obs64 = ot.traces(4).astype(np.float32)
     ot.close()
     log("bench: FP64 oracle gradient")
-    o = _grad_shot(oracle.Oracle, bg, W, W // 2, nt, prec="f64")
+    o = _grad_shot(oracle.Oracle, bg, W, W // 2, nt, form=form, prec="f64")
     J64, g64 = o.gradient(obs_p=obs64)
     o.close()
     w64 = time.perf_counter() - t0
     log("bench: FP32 oracle gradients (shared and own data)")
-    o32 = _grad_shot(oracle.Oracle, bg, W, W // 2, nt, prec="f32")
+    o32 = _grad_shot(oracle.Oracle, bg, W, W // 2, nt, form=form, prec="f32")
     _, g32 = o32.gradient(obs_p=obs64)
-    ot32 = _grad_shot(oracle.Oracle, tr, W, W // 2, nt, prec="f32")
+    ot32 = _grad_shot(oracle.Oracle, tr, W, W // 2, nt, form=form, prec="f32")
     ot32.run()
     _, g32o = o32.gradient(obs_p=ot32.traces(4).astype(np.float32))
     ot32.close()
     o32.close()
     errs = {"reference_fp32_oracle": [round(x, 7) for x in _rel(g32, g64)]}
     own = {"reference_fp32_oracle": [round(x, 7) for x in _rel(g32o, g64)]}
-    gt = _grad_shot(Modeling, tr, W, W // 2, nt)
+    gt = _grad_shot(Modeling, tr, W, W // 2, nt, form=form)
     gt.run()
     obs_gpu = gt.traces(4).copy()
     gt.close()
-    g = _grad_shot(Modeling, bg, W, W // 2, nt)
-    for name in ("plain_f32_checkpoints", "f16_history"):
+    g = _grad_shot(Modeling, bg, W, W // 2, nt, form=form)
+    for name in ("plain_f32_checkpoints", "f16_history", "f16_checkpoints"):
         _, gg = g.gradient(obs_p=obs64, **GRAD_MODES[name])
         errs[name] = [round(x, 7) for x in _rel(gg, g64)]
         _, gg = g.gradient(obs_p=obs_gpu, **GRAD_MODES[name])
         own[name] = [round(x, 7) for x in _rel(gg, g64)]
     g.close()
-    return {"sample": f"FP64 oracle gradient on the 1126 x {W} crop, nt={nt}, Bond form, no free surface, true model "
+    return {"sample": f"FP64 oracle gradient on the 1126 x {W} crop, nt={nt}, {form} form, no free surface, true model "
                       "= background + 5 anomalies inside the crop; ceil(sqrt(nt)) checkpoints, OpenMP adjoint",
             "seconds_fp64_forward_and_gradient": round(w64, 1), "cores": cores, "misfit_fp64": J64,
             "gpu_rel_l2_vs_fp64": errs, "gpu_rel_l2_vs_fp64_own_data": own}
@@ -522,7 +525,7 @@ def main():
         full = synth.model_large()
         out = {"single_shot": gradient_single(full, args.grad_steps, modes, gforms)}
         if args.fp64_c4:
-            out["cpu_fp64_c4"] = cpu_gradient_c4(args.grad_steps)
+            out["cpu_fp64_c4"] = {fm: cpu_gradient_c4(args.grad_steps, fm) for fm in gforms}
         print(json.dumps({"gradients": out}))
         return
     nt = max(args.steps, args.cpu_steps, 1)
