@@ -282,6 +282,26 @@ def test_rsg_exact_on_linear_fields(dx, dz):
     assert np.allclose((o.get("vz") / k)[sl], GAIN * (-b), rtol=1e-10)
 
 
+def test_coordinate_form_dz_of_sxx_carries_invdx():
+    """CR:633: the coordinate form's d(sxx)/dz is scaled by invdx, not invdz.  With the
+    velocity-point trig set to theta = 90 degrees (cosv = 0, sinv = 1) the velocity update
+    is vx = k (-sxxdz + sxzdx): on sxx = b z (sxz = 0) the diagonal pair gives
+    0.5 (d1 - d2) = GAIN b dz, so vx / k = -GAIN b dz / dx (-GAIN b with invdz) -- at
+    dx = 8, dz = 12.5 the two differ by 56%."""
+    dx, dz = 8.0, 12.5
+    o = _sweep_setup(form="coord", dx=dx, dz=dz)
+    Z = np.arange(o.Nz)[None, :] * dz + 0.0 * np.arange(o.Nx)[:, None]
+    b = -1.3
+    o.set("cosv", np.zeros((o.Nx, o.Nz)))
+    o.set("sinv", np.ones((o.Nx, o.Nz)))
+    o.set("sxx", b * Z)
+    o.v_sweep()
+    k = o.get("coef2") * o.get("invrho")
+    sl = _interior(o)
+    assert np.allclose((o.get("vx") / k)[sl], -GAIN * b * dz / dx, rtol=1e-10)
+    assert np.allclose(o.get("vz")[sl], 0.0, atol=1e-12)
+
+
 def _symbol(kappa, h):
     return (2.0 / h) * np.sum(C * np.sin((2 * np.arange(1, 6) - 1) * kappa * h / 2))
 
