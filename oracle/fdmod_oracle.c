@@ -22,7 +22,8 @@
  *   pad_model          numpy edge padding (np.pad mode="edge")
  *   stability          the closed form on a constant model (hand value)
  *   meal_coefs         interior D=0/Beta=1 exactly, Rcoef/d0 hand values at the
- *                      outermost layer, monotone profile, corner = hypot of sides
+ *                      outermost layer, every inner layer against 2^((l/n)^alpha) - 1,
+ *                      monotone profile, corner = hypot of sides
  *   cij_*              Bond: 2-D Bond matrix R C R^T in numpy; Thomsen inversion
  *                      (eps, delta recovered from C11..C55); theta=0 identity
  *   invrho / trig avg  numpy slicing restatement on random tiny grids
@@ -30,9 +31,12 @@
  *                      beyond it (the reference cannot be built here)
  *   v/s sweeps         exactness on linear fields (gain sum (2k-1)C_k), plane-wave
  *                      symbol, cross-form equality at constant theta, analytic 2-D
- *                      Green's function for an isotropic explosive source
- *   sources            closed-form Gaussian weights, linearity
+ *                      Green's function for an isotropic explosive source; the
+ *                      coordinate form's invdx on d(sxx)/dz at dx != dz (CR:633)
+ *   sources            closed-form Gaussian weights (also at dx != dz), linearity
  *   receivers          bit equality with the restated fields
+ *   energy             hand values on constant states; bounded, oscillating kinetic
+ *                      part without damping or source
  */
 #include <math.h>
 #include <stdio.h>
