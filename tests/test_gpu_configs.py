@@ -68,7 +68,7 @@ def test_config1_small_bond_bitwise():
         a = g.traces(comp)
         assert np.isfinite(a).all()
         assert np.array_equal(a, o.traces(comp).astype(np.float32)), comp
-        # the north star's trace tolerance against the FP64 oracle
+        # the 1e-5 relative L2 trace tolerance against the FP64 oracle
         assert rel_l2(a, o64.traces(comp)) <= 1e-5, comp
     for f in ("vx", "vz", "sxx", "szz", "sxz"):
         assert np.array_equal(g.get(f), o.get(f).astype(np.float32)), f
