@@ -160,7 +160,9 @@ int fdmod_batch_set_receivers(fdmod_batch *b, int shot, int seistype, int mt, in
 /* The shot's own context, for per-shot queries (fdmod_dims, fdmod_get_array, ...). */
 fdmod_ctx *fdmod_batch_shot(fdmod_batch *b, int shot);
 /* Autotuner: time the candidate segment lengths of both sweeps on this batch and keep
- * the fastest (reps launches per candidate); resets the wavefields. */
+ * the fastest (reps launches per candidate, after a warm-up); resets the wavefields.
+ * segLV 0 (the default, and a tuner candidate) gives each shot group the shortest
+ * velocity segment whose blocks all run in one generation on the device. */
 int fdmod_batch_tune(fdmod_batch *b, int reps, int *segLV, int *segLS);
 int fdmod_batch_set_tiles(fdmod_batch *b, int segLV, int segLS);
 int fdmod_batch_run(fdmod_batch *b, int k0, int count, double *seconds);

@@ -95,10 +95,11 @@ def test_autotuner_and_tile_shapes_keep_results():
     shots = _shots()
     b = _batch("bond", shots)
     v, s = b.tune(3)
-    assert v in (22, 33, 44, 55, 66, 88) and s in (4, 5, 8, 11, 22)
+    assert v in (0, 22, 33, 44, 55, 66, 88) and s in (4, 5, 8, 11, 22)
     b.run()
     ref = [b.traces(i, 4).copy() for i in range(len(shots))]
-    for lv, ls in ((22, 11), (88, 44), (7, 5)):
+    # 0: per-group velocity segments (the default)
+    for lv, ls in ((22, 11), (88, 44), (7, 5), (0, 11)):
         b.reset()
         b.set_tiles(lv, ls)
         b.run()

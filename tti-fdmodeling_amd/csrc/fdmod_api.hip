@@ -90,6 +90,16 @@ extern "C" void fdmod_destroy(fdmod_ctx *c)
   delete c;
 }
 
+// Velocity-sweep blocks the device holds at once: kVBlocksPerCU per CU.
+static long vsweep_slots()
+{
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                              hipSuccess || cus < 1)
+    cus = 256;
+  return (long)cus * kVBlocksPerCU;
+}
+
 extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
 {
   if (!p || !out) return FDMOD_EINVAL;
@@ -116,11 +126,7 @@ extern "C" int fdmod_create(const fdmod_params *p, fdmod_ctx **out)
   // between kFlagW and kDefaultSegLV columns.  On 1256 x 8002 (256 CUs) that is 38
   // columns, 4009 blocks: Bond 57.4-58.3 Gpts/s against 56.6-57.6 with 44 columns.
   {
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-                                                hipSuccess || cus < 1)
-      cus = 256;
-    const long slots = (long)cus * kVBlocksPerCU;
+    const long slots = vsweep_slots();
     const long want = ((long)g.tilesZ * (g.ixEnd - g.ixStart) + slots - 1) / slots;
     g.segLV = (int)std::min<long>(kDefaultSegLV, std::max<long>(kFlagW, want));
   }
@@ -942,18 +948,35 @@ static int batch_prepare(fdmod_batch *b, bool with_graph)
   std::vector<int> startV(n + ng, 0), startS(n + ng, 0);
   b->totV.assign(ng, 0);
   b->totS.assign(ng, 0);
+  b->grpLV.assign(ng, b->segLV);
+  if (b->segLV == 0) {
+    // per group, as for a single shot: the shortest velocity segment (at least kFlagW
+    // columns) whose blocks, over all the group's shots, run in one generation
+    const long slots = vsweep_slots();
+    for (int g = 0; g < ng; g++) {
+      int L = kFlagW;
+      for (;; L++) {
+        long tiles = 0;
+        for (int s = b->grp[g]; s < b->grp[g + 1]; s++) tiles += sweep_tiles(b->shots[s]->g, L);
+        if (tiles <= slots || L >= kMaxSegLVBatch) break;
+      }
+      b->grpLV[g] = L;
+    }
+  }
   for (int s = 0; s < n; s++) {
     fdmod_ctx *c = b->shots[s];
     if (!c->model_set) return bfail(b, FDMOD_ESTATE, "shot %d has no model", s);
     if (c->wsave > 0 || c->energy) return bfail(b, FDMOD_EINVAL, "shot %d: snapshots/energy are single-shot only", s);
-    c->g.segLV = b->segLV;
-    c->g.segLS = b->segLS;
-    sw[s] = sweep_args(c);
     const int g = (int)(std::upper_bound(b->grp.begin(), b->grp.end(), s) - b->grp.begin()) - 1;
+    // the batch's tiles go into the launch arguments only: the shot's own context keeps
+    // the single-shot tiles for its gradient and single-shot runs
+    sw[s] = sweep_args(c);
+    sw[s].g.segLV = b->grpLV[g];
+    sw[s].g.segLS = b->segLS;
     sw[s].tau_nt = 20.0 * 4.0 * gpts[g] > 512.0 * (1 << 20);  // tau is reused after the group's sweeps
     b->grp_tau_nt[g] = (char)sw[s].tau_nt;
     const int k = s + g;  // this shot's entry in its group's prefix array
-    startV[k + 1] = startV[k] + sweep_tiles(c->g, b->segLV);
+    startV[k + 1] = startV[k] + sweep_tiles(c->g, b->grpLV[g]);
     startS[k + 1] = startS[k] + sweep_tiles(c->g, b->segLS);
     b->totV[g] = startV[k + 1];
     b->totS[g] = startS[k + 1];
@@ -1087,10 +1110,22 @@ extern "C" int fdmod_batch_get_traces(fdmod_batch *b, int shot, int comp, float 
 extern "C" int fdmod_batch_tune(fdmod_batch *b, int reps, int *segLV, int *segLS)
 {
   if (!b || reps < 1) return FDMOD_EINVAL;
-  static const int candV[] = {22, 33, 44, 55, 66, 88};
+  static const int candV[] = {0, 22, 33, 44, 55, 66, 88};  // 0: per group (one generation)
   static const int candS[] = {4, 5, 8, 11, 22};
+  constexpr int kTuneWarmSteps = 64;
   const int n = std::min(reps, b->p.nt);
   const int zero = 0;
+  // Bring the GPU out of idle first: candidates timed straight after the setup read
+  // slow in the order they run (on 64 crops of 1126 x 7872 a cold tuner chose 55/11
+  // columns, 54.4 Gpts/s, a warm one 22/11, 54.7).
+  {
+    int rc = batch_prepare(b, false);
+    if (rc) return rc;
+    for (int done = 0; done < kTuneWarmSteps; done += n) {
+      BHIP(b, hipMemcpyAsync(b->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
+      batch_steps(b, n);
+    }
+  }
   auto time_one = [&](bool vsweep, int L, double &ms_out) -> int {
     if (vsweep) b->segLV = L;
     else b->segLS = L;
@@ -1134,7 +1169,7 @@ extern "C" int fdmod_batch_tune(fdmod_batch *b, int reps, int *segLV, int *segLS
 
 extern "C" int fdmod_batch_set_tiles(fdmod_batch *b, int segLV, int segLS)
 {
-  if (!b || segLV < 1 || segLS < 1 || segLV > 4096 || segLS > 4096) return FDMOD_EINVAL;
+  if (!b || segLV < 0 || segLS < 1 || segLV > 4096 || segLS > 4096) return FDMOD_EINVAL;
   b->segLV = segLV;
   b->segLS = segLS;
   b->prepared = false;

@@ -105,7 +105,9 @@ struct fdmod_batch {
   std::vector<fdmod_ctx *> shots;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int segLV = fdm::kDefaultSegLV, segLS = fdm::kDefaultSegLS;
+  // segLV 0: each shot group's velocity segment is chosen so that its blocks run in one
+  // generation (grpLV); otherwise every group uses segLV
+  int segLV = 0, segLS = fdm::kDefaultSegLS;
   bool prepared = false;
   fdm::SweepArgs *d_sweep = nullptr;
   int *d_startV = nullptr, *d_startS = nullptr, totalV = 0, totalS = 0;
@@ -114,6 +116,7 @@ struct fdmod_batch {
   // its tile totals are totV/totS[g].  Groups hold up to group_pts padded points.
   std::vector<int> grp, totV, totS;
   std::vector<char> grp_tau_nt;  // per group: tensor-form tau stored non-temporally
+  std::vector<int> grpLV;        // per group: velocity-sweep segment length
   size_t group_pts = fdm::kBatchGroupPts;
   fdm::SourceArgs *d_srcV = nullptr, *d_srcS = nullptr;
   int srcBlocksV = 0, srcBlocksS = 0;

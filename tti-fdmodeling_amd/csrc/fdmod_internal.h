@@ -30,6 +30,7 @@ constexpr int kRing = 11;          // LDS ring of staged columns (10 live + 1 in
 constexpr int kFlagW = 11;         // columns per fine flag tile
 constexpr int kDefaultSegLV = 44;  // longest velocity-sweep segment
 constexpr int kVBlocksPerCU = 16;  // velocity-sweep blocks per CU: 160 KB LDS / 9.6 KB ring
+constexpr int kMaxSegLVBatch = 88;  // longest per-group velocity segment of a batched loop
 constexpr int kTZS = 55;           // stress adjoint: output rows per strip (64 staged rows, halo 4 + 5)
 // Stress-sweep segments: 11 columns on every grid.  Odd segments walk right to left
 // and share their halo through L2.  Measured on 1256 x 8002 with 128-byte-aligned
