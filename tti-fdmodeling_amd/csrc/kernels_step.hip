@@ -83,11 +83,14 @@ __device__ __forceinline__ void load_vops(VOps &p, const SweepArgs &a, size_t o,
 // Velocity sweep (A1 + A3).  Bond and tensor: BT:654-704 / TR:617-666.
 // Coordinate (COORD): CR:614-671, with the reference's invdx on d(sxx)/dz.
 // The loads of the staged column and of the per-point operands run PD iterations
-// ahead of their use.
-template <bool COORD, int PD, int ESTORE = 0, bool OPNT = true>
-__device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx)
+// ahead of their use.  REV walks the segment right to left (odd segments), as the
+// stress sweep does, so that neighbouring segments stage the stress columns they
+// share at the same time and the second load finds them in L2.  Column xa+j
+// (forward) or xe-j (reversed) lives in ring slot (j+4) or (j+5) mod kRing.  The
+// ring is the caller's, so that both directions share one LDS array.
+template <bool COORD, int PD, int ESTORE = 0, bool OPNT = true, bool REV = false>
+__device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx, float (*ring)[kRing][LZ])
 {
-  __shared__ float ring[3][kRing][LZ];
   const Grid &g = a.g;
   const int segL = g.segLV;
   const int lane = threadIdx.x;
@@ -101,11 +104,12 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
   const int xe = xa + nc - 1;
   const int rs = z0 - 4;  // first staged row
   const float *const fs[3] = {a.sxx, a.szz, a.sxz};
-  // prologue: columns xa-4 .. xa+4 into slots 0..8; the next PD columns and operands in flight
+  // prologue: columns xa-4 .. xa+4 (reversed: xe+5 down to xe-3) into slots 0..8; the
+  // next PD columns and operands in flight
   {
     Col<3> c[9];
 #pragma unroll
-    for (int m = 0; m < 9; m++) load_col<3>(c[m], fs, xa - 4 + m, rs, lane, g);
+    for (int m = 0; m < 9; m++) load_col<3>(c[m], fs, REV ? xe + 5 - m : xa - 4 + m, rs, lane, g);
 #pragma unroll
     for (int m = 0; m < 9; m++) store_col<3>(ring, m, c[m], lane);
   }
@@ -113,8 +117,8 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
   VOps opq[PD];
 #pragma unroll
   for (int i = 0; i < PD; i++) {
-    if (i <= nc - 1) load_col<3>(colq[i], fs, xa + 5 + i, rs, lane, g);
-    if (zok && i < nc) load_vops<COORD, OPNT>(opq[i], a, (size_t)(xa + i) * g.ldz + iz, und, g.dt);
+    if (i <= nc - 1) load_col<3>(colq[i], fs, REV ? xe - 4 - i : xa + 5 + i, rs, lane, g);
+    if (zok && i < nc) load_vops<COORD, OPNT>(opq[i], a, (size_t)(REV ? xe - i : xa + i) * g.ldz + iz, und, g.dt);
   }
   const float invdx = g.invdx, invdz = g.invdz;
   for (int k0 = 0; k0 < nc; k0 += kRing) {
@@ -122,26 +126,27 @@ __device__ __forceinline__ void vsweep_tile(const SweepArgs &a, int btz, int btx
       constexpr int u = decltype(uc)::value;
       const int k = k0 + u;
       if (k >= nc) return;
-      const int c = xa + k;
+      const int c = REV ? xe - k : xa + k;
       Col<3> cn;
       VOps on;
       // look-ahead loads with no branch around them (clamped column and row; the values
       // past the segment or the grid are not used)
-      load_col_nb<3>(cn, fs, min(c + 5 + PD, xe + 5), rs, lane, g);
-      load_vops<COORD, OPNT>(on, a, (size_t)min(c + PD, xe) * g.ldz + izc, und, g.dt);
+      load_col_nb<3>(cn, fs, REV ? max(c - 4 - PD, xa - 4) : min(c + 5 + PD, xe + 5), rs, lane, g);
+      load_vops<COORD, OPNT>(on, a, (size_t)(REV ? max(c - PD, xa) : min(c + PD, xe)) * g.ldz + izc, und, g.dt);
       store_col<3>(ring, (u + 9) % kRing, colq[0], lane);
       __syncthreads();
+      auto slot = [](int dx) { return REV ? (u - dx + 5 + kRing) % kRing : (u + dx + 4 + kRing) % kRing; };
       float hv[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // coordinate-form history (ESTORE)
       if (zok) {
         const VOps &op = opq[0];
         float d1, d2;
-        rsg_fwd([&](int dx, int dz) { return ring[0][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        rsg_fwd([&](int dx, int dz) { return ring[0][slot(dx)][lane + 4 + dz]; }, d1, d2);
         const float sxxdx = 0.5f * (d1 + d2) * invdx;
         const float sxxdz = COORD ? 0.5f * (d1 - d2) * invdx : 0.0f;
-        rsg_fwd([&](int dx, int dz) { return ring[1][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        rsg_fwd([&](int dx, int dz) { return ring[1][slot(dx)][lane + 4 + dz]; }, d1, d2);
         const float szzdx = COORD ? 0.5f * (d1 + d2) * invdx : 0.0f;
         const float szzdz = 0.5f * (d1 - d2) * invdz;
-        rsg_fwd([&](int dx, int dz) { return ring[2][(u + dx + 4 + kRing) % kRing][lane + 4 + dz]; }, d1, d2);
+        rsg_fwd([&](int dx, int dz) { return ring[2][slot(dx)][lane + 4 + dz]; }, d1, d2);
         const float sxzdx = 0.5f * (d1 + d2) * invdx;
         const float sxzdz = 0.5f * (d1 - d2) * invdz;
         const size_t o = (size_t)c * g.ldz + iz;
@@ -481,9 +486,17 @@ constexpr int pipe_s() { return FDMOD_PIPE_S; }
 template <bool COORD, int PD, int ESTORE = 0, bool OPNT = true>
 __global__ __launch_bounds__(64) void k_vsweep(SweepArgs a)
 {
+  __shared__ float ring[3][kRing][LZ];
   int btz, btx;
   if (!tile_of_block(a.g, a.g.segLV, btz, btx)) return;
-  vsweep_tile<COORD, PD, ESTORE, OPNT>(a, btz, btx);
+  // right-to-left odd segments only on grids that stream past the last-level cache:
+  // there neighbours stage the columns they share together (1256 x 8002, coordinate form:
+  // velocity sweep 71.6-72.1 -> 70.1-70.6 us, loop 59.2-59.4 -> 59.7-60.0 Gpts/s; Bond
+  // 60.6-61.0 -> 60.9-61.2, tensor unchanged); on cache-resident grids they
+  // saved nothing and the coordinate FP16-history gradient on a 1126 x 2000 crop ran 1%
+  // slower (0.629 -> 0.635 s)
+  if (OPNT && (btx & 1)) vsweep_tile<COORD, PD, ESTORE, OPNT, true>(a, btz, btx, ring);
+  else vsweep_tile<COORD, PD, ESTORE, OPNT, false>(a, btz, btx, ring);
 }
 
 template <int FORM, int PD, int ESTORE = 0, bool TNT = false, bool OPNT = true>
@@ -528,11 +541,13 @@ template <bool COORD, int PD>
 __global__ __launch_bounds__(64) void k_vsweep_batch(const SweepArgs *__restrict__ shots, const int *__restrict__ start,
                                                      int nshots)
 {
+  __shared__ float ring[3][kRing][LZ];
   int s, lt;
   if (!batch_tile(start, nshots, s, lt)) return;
   const SweepArgs a = shots[s];
   const int btx = lt / a.g.tilesZ;
-  vsweep_tile<COORD, PD>(a, lt - btx * a.g.tilesZ, btx);
+  if (btx & 1) vsweep_tile<COORD, PD, 0, true, true>(a, lt - btx * a.g.tilesZ, btx, ring);
+  else vsweep_tile<COORD, PD, 0, true, false>(a, lt - btx * a.g.tilesZ, btx, ring);
 }
 
 // Shots with a fused stress source or pressure receivers (as in the single-shot loop)
