@@ -103,6 +103,30 @@ def test_traces_bitwise_dx_ne_dz(form, fs, stype):
 
 # snapshots (isoelastic.c:240-289): savew = 1 keeps the absorbing layers (the whole
 # padded grid), savew = 0 crops them; vx and vz as they are, p = 0.5 (sxx + szz)
+@pytest.mark.parametrize("form", ["bond", "tensor", "coord"])
+@pytest.mark.parametrize("fs", [0, 1])
+def test_pressure_receivers_after_unfused_stress_sources(form, fs):
+    # three stype-1 sources with overlapping footprints, which the stress sweep does not
+    # add itself, and pressure receivers through the footprints: the receivers sample
+    # after the injection, in the reference's step order (BT:893-924)
+    o, g, _ = _pair(form=form, freesurface=fs, nt=300, seistype=2, nsrc=3, sz=3, rz=3)
+    o.run()
+    g.run()
+    assert np.array_equal(g.traces(4), o.traces(4).astype(np.float32))
+
+
+@pytest.mark.parametrize("form", ["bond", "tensor", "coord"])
+def test_edge_source_with_receivers_in_its_footprint(form):
+    # one stype-1 source within five cells of the grid's left edge (not added by the
+    # stress sweep): one launch adds it and records the receivers, seven of them inside
+    # its footprint
+    o, g, _ = _pair(form=form, nt=300, seistype=3, ealThick=3, sx=1, sz=3, rz=3)
+    o.run()
+    g.run()
+    for comp in (1, 3, 4):
+        assert np.array_equal(g.traces(comp), o.traces(comp).astype(np.float32)), comp
+
+
 @pytest.mark.parametrize("savew", [0, 1])
 def test_snapshots_bitwise(savew):
     nt, every = 120, 40

@@ -73,7 +73,7 @@ static void release(fdmod_ctx *c)
   dfree(c->pool); dfree(c->flags);
   dfree(c->d_isx); dfree(c->d_isz); dfree(c->d_src); dfree(c->d_taps);
   dfree(c->d_gx); dfree(c->d_gz); dfree(c->d_rvx); dfree(c->d_rvz); dfree(c->d_rp);
-  dfree(c->d_rc_start); dfree(c->d_rc_z); dfree(c->d_rc_id);
+  dfree(c->d_rc_start); dfree(c->d_rc_z); dfree(c->d_rc_id); dfree(c->d_fprc);
   dfree(c->d_p); dfree(c->d_E); dfree(c->d_Ek); dfree(c->d_Ep); dfree(c->d_partial);
   dfree(c->d_mu); dfree(c->d_mus); dfree(c->d_G); dfree(c->d_snap); dfree(c->d_res); dfree(c->d_red);
   dfree(c->ckpt.data); dfree(c->ckpt.scale); dfree(c->hist.data); dfree(c->hist.scale);
@@ -319,6 +319,27 @@ static void build_taps(const fdmod_params &p, int stype, std::vector<SrcTap> &ta
   }
 }
 
+// The receivers inside a single stype-1 source's footprint (the cells its taps write),
+// for the combined source + receiver launch; nfprc = -1 where that launch does not apply.
+static int fp_update(fdmod_ctx *c)
+{
+  dfree(c->d_fprc);
+  c->d_fprc = nullptr;
+  c->nfprc = -1;
+  if (c->ns != 1 || c->stype != 1 || c->seistype == 0 || c->nr == 0 || c->h_gx.size() != (size_t)c->nr ||
+      c->h_isx.size() != 1)
+    return FDMOD_OK;
+  std::vector<int> in;
+  for (int i = 0; i < c->nr; i++)
+    if (abs(c->h_gx[i] - c->h_isx[0]) <= kN && abs(c->h_gz[i] - c->h_isz[0]) <= kN) in.push_back(i);
+  if (!in.empty()) {
+    HIPCALL(c, hipMalloc(&c->d_fprc, in.size() * sizeof(int)));
+    HIPCALL(c, hipMemcpy(c->d_fprc, in.data(), in.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  c->nfprc = (int)in.size();
+  return FDMOD_OK;
+}
+
 extern "C" int fdmod_set_source(fdmod_ctx *c, int stype, int ns, const int *posx, const int *posz, const float *src)
 {
   if (!c) return FDMOD_EINVAL;
@@ -340,6 +361,9 @@ extern "C" int fdmod_set_source(fdmod_ctx *c, int stype, int ns, const int *posx
     isx[i] = posx[i] + c->g.e;
     isz[i] = posz[i] + c->g.e;
   }
+  c->h_isx = isx;
+  c->h_isz = isz;
+  c->nfprc = -1;
   // sources may run in parallel only if no two footprints share a cell
   int rad = stype == 8 ? 1 : kN;
   std::vector<int> order(ns);
@@ -378,7 +402,7 @@ extern "C" int fdmod_set_source(fdmod_ctx *c, int stype, int ns, const int *posx
   HIPCALL(c, hipMemcpy(c->d_isz, isz.data(), ns * sizeof(int), hipMemcpyHostToDevice));
   HIPCALL(c, hipMemcpy(c->d_src, src, (size_t)ns * p.nt * sizeof(float), hipMemcpyHostToDevice));
   HIPCALL(c, hipMemcpy(c->d_taps, taps.data(), taps.size() * sizeof(SrcTap), hipMemcpyHostToDevice));
-  return FDMOD_OK;
+  return fp_update(c);
 }
 
 extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, const int *posx, const int *posz)
@@ -405,12 +429,17 @@ extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, c
   c->nr = nr;
   c->rline = false;
   c->ntsample = (p.nt - 1) / mt + 1;  // obsinfo.c:199
+  c->nfprc = -1;
+  c->h_gx.clear();
+  c->h_gz.clear();
   if (nr == 0 || seistype == 0) return FDMOD_OK;
   std::vector<int> gx(nr), gz(nr);
   for (int i = 0; i < nr; i++) {
     gx[i] = posx[i] + c->g.e;
     gz[i] = posz[i] + c->g.e;
   }
+  c->h_gx = gx;
+  c->h_gz = gz;
   size_t nrec = (size_t)nr * c->ntsample * sizeof(float);
   HIPCALL(c, hipMalloc(&c->d_gx, nr * sizeof(int)));
   HIPCALL(c, hipMalloc(&c->d_gz, nr * sizeof(int)));
@@ -457,7 +486,7 @@ extern "C" int fdmod_set_receivers(fdmod_ctx *c, int seistype, int mt, int nr, c
     HIPCALL(c, hipMemcpy(c->d_rc_id, rid.data(), nr * sizeof(int), hipMemcpyHostToDevice));
   }
   HIPCALL(c, hipStreamSynchronize(c->stream));
-  return FDMOD_OK;
+  return fp_update(c);
 }
 
 extern "C" int fdmod_set_snapshots(fdmod_ctx *c, int wsave, int savew, int snap_sample, fdmod_snapshot_fn fn,
@@ -579,7 +608,12 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record, const hipEvent_
   const bool snap_now = record && c->wsave > 0 && it % c->snap_sample == 0;
   const bool rec_now = record && c->seistype > 0 && c->nr > 0 && it % c->mt == 0;
   const bool fsrc = c->fuse_src && c->ns > 0;
-  const bool frec = rec_now && c->d_rc_start && p.form != FDMOD_FORM_TENSOR;
+  // The receivers sample after the stress source (BT:893-924), so the sweep records them
+  // only when it also adds that source, or when there is none.
+  const bool src1 = c->ns > 0 && c->stype == 1;
+  const bool frec = rec_now && c->d_rc_start && p.form != FDMOD_FORM_TENSOR && (fsrc || !src1);
+  // otherwise one launch can add a single stype-1 source and record the receivers
+  const bool combine = rec_now && !frec && src1 && !fsrc && c->nfprc >= 0;
   if (fsrc || frec) {  // the stress sweep adds the source and/or records the receivers
     SweepArgs sf = sa;
     if (fsrc) {
@@ -615,21 +649,22 @@ int step(fdmod_ctx *c, int it, const SweepArgs &sa, bool record, const hipEvent_
     r.want_p = c->seistype == 2 || c->seistype == 3;
     r.mt = c->mt;
   }
-  if (c->ns > 0 && c->stype == 1 && !fsrc) {
+  if (src1 && !fsrc) {
     src.fa = sa.sxx;
     src.fb = sa.szz;
     if (p.form == FDMOD_FORM_TENSOR) {  // TR:888-891
       src.fc = sa.txx;
       src.fd = sa.tzz;
     }
-    launch_source(c->stream, src);
+    if (combine) launch_source_receivers(c->stream, src, r, c->d_fprc, c->nfprc);
+    else launch_source(c->stream, src);
   }
   if (!record) return FDMOD_OK;
   if (snap_now) {
     int rc = snapshot(c, it);
     if (rc) return rc;
   }
-  if (rec_now && !frec) launch_receivers(c->stream, r);
+  if (rec_now && !frec && !combine) launch_receivers(c->stream, r);
   if (c->energy) {
     launch_energy_partial(c->stream, c->g, p.nx, p.nz, c->d, c->d_partial, c->eblocks);
     launch_energy_final(c->stream, c->d_partial, c->eblocks, p.dx, p.dz, c->d_E, c->d_Ek, c->d_Ep, it);
@@ -985,7 +1020,8 @@ static int batch_prepare(fdmod_batch *b, bool with_graph)
     // as in the single-shot loop, the stress sweep adds a single covered stype-1 source
     // and records seistype-2 receivers; the sweep reads the iteration on the device
     const bool fsrc = c->fuse_src && c->ns > 0;
-    const bool frec = c->seistype == 2 && c->nr > 0 && c->d_rc_start && b->p.form != FDMOD_FORM_TENSOR;
+    const bool frec = c->seistype == 2 && c->nr > 0 && c->d_rc_start && b->p.form != FDMOD_FORM_TENSOR &&
+                      (fsrc || !(c->ns > 0 && c->stype == 1));  // receivers sample after the stress source
     if (fsrc) {
       sw[s].fuse |= 1;
       sw[s].fsx = c->src_cell_x;

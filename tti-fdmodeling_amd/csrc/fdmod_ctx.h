@@ -47,6 +47,12 @@ struct fdmod_ctx {
   bool rline = false;
   int rline_x0 = 0, rline_z = 0;
   const float *inj_last = nullptr;  // the last adjoint step's fused residuals (or null)
+  // one unfused stype-1 source: its receivers inside the source's footprint (indices),
+  // which the combined source + receiver launch records after the injection; nfprc < 0:
+  // no combined launch.  Host copies of the padded source and receiver cells build it.
+  int *d_fprc = nullptr;
+  int nfprc = -1;
+  std::vector<int> h_isx, h_isz, h_gx, h_gz;
   // snapshots
   int wsave = 0, savew = 0, snap_sample = 1;
   fdmod_snapshot_fn snap_fn = nullptr;

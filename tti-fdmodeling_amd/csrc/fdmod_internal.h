@@ -235,6 +235,10 @@ void launch_receivers_batch(hipStream_t s, const RecvArgs *shots, int nshots, in
 void launch_advance(hipStream_t s, int *step, int n);
 void launch_source(hipStream_t s, const SourceArgs &a);
 void launch_receivers(hipStream_t s, const RecvArgs &a);
+// One stype-1 source (ns = 1) and the receivers in one launch: block 0 injects the source,
+// then records the nfp receivers fprc[] inside its footprint; the other blocks record
+// every other receiver.
+void launch_source_receivers(hipStream_t s, const SourceArgs &src, const RecvArgs &r, const int *fprc, int nfp);
 void launch_pressure(hipStream_t s, const float *sxx, const float *szz, float *p, int Nx, int ldz);
 void launch_energy_partial(hipStream_t s, const Grid &g, int nx, int nz, const DevArrays &d, float *partial,
                            int nblocks);

@@ -727,6 +727,30 @@ void launch_receivers(hipStream_t s, const RecvArgs &a)
   hipLaunchKernelGGL(k_receivers, dim3((a.nr + 255) / 256), dim3(256), 0, s, a);
 }
 
+// Stress source and receivers in one launch (one stype-1 source that the stress sweep does
+// not add itself).  Block 0 adds the source tap by tap as k_source does, then, after the
+// barrier that ends the injection, records the receivers inside the source's footprint
+// (listed on the host).  Blocks 1.. record every other receiver: their cells are outside
+// the footprint, so the injection leaves them unchanged and the two run concurrently.
+__global__ void k_source_receivers(SourceArgs src, RecvArgs r, const int *__restrict__ fprc, int nfp)
+{
+  if (blockIdx.x == 0) {
+    source_body(src, src.it, 0);
+    for (int j = threadIdx.x; j < nfp; j += blockDim.x) receiver_body(r, r.isample, fprc[j]);
+    return;
+  }
+  const int i = (blockIdx.x - 1) * blockDim.x + threadIdx.x;
+  if (i >= r.nr) return;
+  const int fx = src.isx[0], fz = src.isz[0];
+  if (abs(r.gx[i] - fx) <= kN && abs(r.gz[i] - fz) <= kN) return;  // block 0's
+  receiver_body(r, r.isample, i);
+}
+
+void launch_source_receivers(hipStream_t s, const SourceArgs &src, const RecvArgs &r, const int *fprc, int nfp)
+{
+  hipLaunchKernelGGL(k_source_receivers, dim3(1 + (r.nr + 255) / 256), dim3(256), 0, s, src, r, fprc, nfp);
+}
+
 // Batched receivers: blockIdx.y = shot; records when it % mt == 0 (isoelastic.c:107).
 __global__ void k_receivers_batch(const RecvArgs *__restrict__ shots, const int *step, int off, int nt, int mode)
 {
