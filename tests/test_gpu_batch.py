@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 NZ, NT, DT = 40, 150, 1e-3
 
 
-def _shots():
+def _shots(multi1=False):
     m = synth.model_small(nz=NZ, nx=200, seed=7)
     m["theta"] = np.where(np.arange(NZ)[None, :] > 20, 25.0, -10.0).astype(np.float32) + 0 * m["theta"]
     crops = [(10, 50), (40, 73), (90, 61), (100, 88)]  # (x0, width): uneven apertures
@@ -19,7 +19,8 @@ def _shots():
     shots = []
     for s, (x0, w) in enumerate(crops):
         mc = synth.crop(m, x0, w)
-        ns = 2 if stypes[s] == 8 else 1
+        # multi1: the last shot has two stype-1 sources (not added by the stress sweep)
+        ns = 2 if stypes[s] == 8 or (multi1 and s == 3) else 1
         sx = [w // 2 + 2 * i for i in range(ns)]
         sz = [3 + i for i in range(ns)]
         wav = np.stack([synth.ricker(15.0, DT, NT) * (1 + 0.3 * i) for i in range(ns)])
@@ -89,6 +90,21 @@ def test_batch_pressure_receivers_equal_single_shots(form):
         tb = b.traces(s, 4)
         assert np.array_equal(tb, g.traces(4)), s
         assert np.array_equal(tb, o.traces(4).astype(np.float32)), s
+
+
+# a shot with two stype-1 sources next to shots whose sweeps add theirs: its pressure
+# receivers inside the footprints sample after the separate injection
+@pytest.mark.parametrize("form", ["bond", "tensor", "coord"])
+def test_batch_pressure_receivers_after_unfused_stress_sources(form):
+    shots = _shots(multi1=True)
+    print("batch p receivers, two stype-1 sources in shot 3", form, flush=True)
+    b = _batch(form, shots, seis=2)
+    b.run(0, 70)
+    b.run(70, 80)
+    for s, sh in enumerate(shots):
+        o = _single(Oracle, form, sh, seis=2)
+        o.run()
+        assert np.array_equal(b.traces(s, 4), o.traces(4).astype(np.float32)), s
 
 
 def test_autotuner_and_tile_shapes_keep_results():
