@@ -127,6 +127,25 @@ def test_edge_source_with_receivers_in_its_footprint(form):
         assert np.array_equal(g.traces(comp), o.traces(comp).astype(np.float32)), comp
 
 
+@pytest.mark.parametrize("form", ["bond", "tensor", "coord"])
+@pytest.mark.parametrize("fs", [0, 1])
+def test_pressure_receivers_on_vertical_lines(form, fs):
+    # two full-depth receiver columns (61 receivers each, more than a strip's 64 lanes
+    # hold per segment with the surface line) and a surface line: the stress sweep walks
+    # each column's receiver list in the loop
+    ctor, m, geom = small_case(form=form, freesurface=fs, nt=300, seistype=2, mt=1, sx=40, sz=2)
+    nz, nx = ctor["nz"], ctor["nx"]
+    zs = np.arange(0, nz + 1)
+    geom["rpx"] = np.concatenate([np.full(nz + 1, 30), np.full(nz + 1, 44), np.arange(0, nx + 1)])
+    geom["rpz"] = np.concatenate([zs, zs, np.full(nx + 1, 2)])
+    print("case vsp", form, fs, len(geom["rpx"]), "receivers", flush=True)
+    o = run_case(Oracle, ctor, m, geom)
+    g = run_case(Modeling, ctor, m, geom)
+    o.run()
+    g.run()
+    assert np.array_equal(g.traces(4), o.traces(4).astype(np.float32))
+
+
 @pytest.mark.parametrize("savew", [0, 1])
 def test_snapshots_bitwise(savew):
     nt, every = 120, 40
