@@ -58,6 +58,37 @@ def test_dot_product_and_adjoint_source(fs, stype, seis, mode, mt, form):
     assert rel_l2(a_gpu, a_ref) <= 1e-4
 
 
+# three stype-1 sources with overlapping footprints (injected after the stress sweep) and
+# pressure receivers inside them: the adjoint of each source time function against the
+# FP64 oracle, and the dot product
+@pytest.mark.parametrize("form", FORMS)
+def test_adjoint_source_of_overlapping_stress_sources(form):
+    print("dot, three stype-1 sources", form, flush=True)
+    m = _model()
+    src = synth.ricker(15.0, DT, NT)
+    sw = np.stack([src * (1.0 + 0.2 * i) for i in range(3)])
+    spx, spz = [26, 29, 32], [4, 5, 6]
+
+    def mk(cls, **kw):
+        o = cls(form, NZ, NX, NT, 10.0, 10.0, DT, ealThick=10, freesurface=0, fpeak=15.0, **kw)
+        o.set_model(m["vp"], m["vs"], m["rho"], m["eps"], m["delta"], m["theta"])
+        o.set_source(1, spx, spz, sw)
+        rx = np.arange(0, NX, 2)
+        o.set_receivers(2, 1, rx, np.full(len(rx), 3))
+        return o
+
+    g = mk(Modeling)
+    rng = np.random.default_rng(21)
+    y = rng.standard_normal((g.nr, g.ntsample)).astype(np.float32)
+    lhs, rhs = g.adjoint_dot(None, None, y)
+    assert abs(lhs - rhs) <= 5e-5 * abs(lhs), (lhs, rhs)
+    a_gpu = g.adjoint_source(None, None, y)
+    o = mk(Oracle, prec="f64")
+    a_ref = o.adjoint_source(None, None, y.astype(np.float64), ns=3)
+    assert a_gpu.shape == (3, NT)
+    assert rel_l2(a_gpu, a_ref) <= 1e-4
+
+
 def _obs(fs, seis, mode=1, mt=1, form="bond", **kw):
     m = _model()
     true = {k: v.copy() for k, v in m.items()}
