@@ -146,6 +146,24 @@ def test_pressure_receivers_on_vertical_lines(form, fs):
     assert np.array_equal(g.traces(4), o.traces(4).astype(np.float32))
 
 
+@pytest.mark.parametrize("form", ["bond", "tensor", "coord"])
+@pytest.mark.parametrize("fs", [0, 1])
+@pytest.mark.parametrize("seis", [2, 3])
+def test_stress_source_across_a_strip_boundary(form, fs, seis):
+    # the source's 11-row footprint straddles the boundary between the first two 64-row
+    # strips (padded row izStart + 64), and the receivers lie on the source's row: each
+    # strip adds its own rows of taps
+    izs = (10 + 5) if fs else 5
+    sz = izs + 64 - 15  # padded row izStart + 64 = the source cell
+    o, g, _ = _pair(form=form, freesurface=fs, nz=80, nt=300, seistype=seis, sz=sz, rz=sz)
+    o.run()
+    g.run()
+    for comp in ((4,) if seis == 2 else (1, 3, 4)):
+        assert np.array_equal(g.traces(comp), o.traces(comp).astype(np.float32)), comp
+    for f in ("vx", "vz", "sxx", "szz", "sxz"):
+        assert np.array_equal(g.get(f), o.get(f).astype(np.float32)), f
+
+
 @pytest.mark.parametrize("savew", [0, 1])
 def test_snapshots_bitwise(savew):
     nt, every = 120, 40
